@@ -1,0 +1,294 @@
+"""Python binding of the MI355X YCSB transaction-execution library (libgcctb.so).
+
+A thin ctypes layer over the C ABI declared in include/gcctb.h, with numpy
+buffers.  It mirrors the reference driver's knobs (cc_scheme, warp_density,
+block_size, batch_size; benchmark_cpu/ycsb.cu:90-98).  There is no CPU
+fallback: without the built library or without a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libgcctb.so")
+
+SCHEMES = ("tpl_nw", "tpl_wd", "to", "mvcc", "silo", "tictoc", "gacco", "gputx")
+SEED = 141919810  # the reference's SEED (benchmark_cpu/ycsb.cu:27, generation/gen_ycsb.cu:12)
+
+GCCTB_OK = 0
+
+
+class GcctbError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gcctb error {code}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("cc_scheme", C.c_char_p),
+        ("warp_density", C.c_int32),
+        ("block_size", C.c_int32),
+        ("batch_size", C.c_uint64),
+        ("rows", C.c_uint64),
+        ("cols", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("backoff_ticks", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("txs", C.c_uint64), ("committed", C.c_uint64), ("aborts", C.c_uint64),
+        ("ts_ticks", C.c_uint64), ("wait_ticks", C.c_uint64), ("abort_ticks", C.c_uint64),
+        ("manager_ticks", C.c_uint64), ("tot_ticks", C.c_uint64), ("index_ticks", C.c_uint64),
+        ("tick_khz", C.c_double), ("pre_ms", C.c_double), ("proc_ms", C.c_double),
+        ("tot_ms", C.c_double), ("exec_kernel_ms", C.c_double),
+        ("batches", C.c_uint64), ("ranks", C.c_uint64), ("same_rank_conflicts", C.c_uint64),
+        ("index_miss", C.c_uint64), ("error_flags", C.c_uint64), ("total_ops", C.c_uint64),
+        ("read_ops", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Workload(C.Structure):
+    _fields_ = [
+        ("rows", C.c_uint64), ("cols", C.c_uint32), ("ntx", C.c_uint64), ("tx_base", C.c_uint64),
+        ("len_min", C.c_uint32), ("len_max", C.c_uint32), ("theta", C.c_double),
+        ("read_ratio", C.c_double), ("seed", C.c_uint64), ("threads", C.c_int32),
+    ]
+
+
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Load libgcctb.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GcctbError(-3, f"{LIB_PATH} is missing; build it with gpu_cc_tb_amd/build.py")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "gcctb_create": (C.c_int, [C.POINTER(Config), C.POINTER(P)]),
+        "gcctb_destroy": (None, [P]),
+        "gcctb_last_error": (C.c_char_p, []),
+        "gcctb_load_table": (C.c_int, [P, _u64p, C.c_uint64, C.c_uint32]),
+        "gcctb_seed_table": (C.c_int, [P]),
+        "gcctb_build_index_identity": (C.c_int, [P]),
+        "gcctb_build_index": (C.c_int, [P, _u32p, _u32p, C.c_uint64]),
+        "gcctb_probe": (C.c_int, [P, _u32p, C.c_uint64, _u32p]),
+        "gcctb_load_ops": (C.c_int, [P, _u64p, _u64p, C.c_uint64]),
+        "gcctb_run_plain": (C.c_int, [P, C.POINTER(Stats)]),
+        "gcctb_fetch": (C.c_int, [P, P, P, P, P]),
+        "gcctb_fetch_ranks": (C.c_int, [P, P, P, C.POINTER(C.c_uint64)]),
+        "gcctb_fetch_cc": (C.c_int, [P, _u64p]),
+        "gcctb_generate": (C.c_int, [C.POINTER(Workload), _u64p, _u64p, C.POINTER(C.c_uint64)]),
+        "gcctb_zeta": (C.c_double, [C.c_uint64, C.c_double]),
+        "gcctb_read_txs_file": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p,
+                                          C.POINTER(C.c_uint64)]),
+        "gcctb_write_txs_file": (C.c_int, [C.c_char_p, _u64p, _u64p, C.c_uint64]),
+        "gcctb_device_info": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != GCCTB_OK:
+        raise GcctbError(rc, load_library().gcctb_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# host-only helpers (no GPU needed)
+# ---------------------------------------------------------------------------
+@dataclass
+class Ops:
+    """Transactions in CSR form: ops[tx_off[t]:tx_off[t+1]] belong to transaction t."""
+    ops: np.ndarray
+    tx_off: np.ndarray
+
+    @property
+    def ntx(self) -> int:
+        return len(self.tx_off) - 1
+
+    @property
+    def nops(self) -> int:
+        return int(self.tx_off[-1])
+
+
+def generate(rows: int, ntx: int, *, theta: float, read_ratio: float, len_min: int = 16,
+             len_max: int | None = None, cols: int = 10, seed: int = SEED, tx_base: int = 0,
+             threads: int = 0) -> Ops:
+    """Deterministic YCSB transactions (Zipf keys, distinct and sorted per transaction)."""
+    lib = load_library()
+    len_max = len_min if len_max is None else len_max
+    w = Workload(rows, cols, ntx, tx_base, len_min, len_max, theta, read_ratio, seed, threads)
+    ops = np.zeros(max(1, ntx * len_max), dtype=np.uint64)
+    off = np.zeros(ntx + 1, dtype=np.uint64)
+    n = C.c_uint64(0)
+    _check(lib.gcctb_generate(C.byref(w), ops, off, C.byref(n)))
+    return Ops(ops[: n.value].copy() if n.value else np.zeros(0, dtype=np.uint64), off)
+
+
+def zeta(n: int, theta: float) -> float:
+    return load_library().gcctb_zeta(n, theta)
+
+
+def read_txs(path: str, ntx: int, cols: int = 10, seed: int = SEED) -> Ops:
+    lib = load_library()
+    ops = np.zeros(max(1, 16 * ntx), dtype=np.uint64)
+    off = np.zeros(ntx + 1, dtype=np.uint64)
+    n = C.c_uint64(0)
+    _check(lib.gcctb_read_txs_file(path.encode(), ntx, cols, seed, ops, off, C.byref(n)))
+    return Ops(ops[: n.value].copy(), off)
+
+
+def write_txs(path: str, w: Ops) -> None:
+    _check(load_library().gcctb_write_txs_file(path.encode(), np.ascontiguousarray(w.ops),
+                                               np.ascontiguousarray(w.tx_off), w.ntx))
+
+
+def op_key(ops: np.ndarray) -> np.ndarray:
+    return (ops & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def op_col(ops: np.ndarray) -> np.ndarray:
+    return ((ops >> np.uint64(32)) & np.uint64(0xFF)).astype(np.uint32)
+
+
+def op_is_write(ops: np.ndarray) -> np.ndarray:
+    return ((ops >> np.uint64(40)) & np.uint64(1)).astype(bool)
+
+
+def make_ops(keys, cols, writes) -> np.ndarray:
+    keys = np.asarray(keys, dtype=np.uint64)
+    cols = np.asarray(cols, dtype=np.uint64)
+    writes = np.asarray(writes, dtype=np.uint64)
+    return keys | (cols << np.uint64(32)) | (writes << np.uint64(40))
+
+
+def device_info() -> dict:
+    lib = load_library()
+    name = C.create_string_buffer(256)
+    khz = C.c_double(0)
+    cus = C.c_int32(0)
+    _check(lib.gcctb_device_info(name, 256, C.byref(khz), C.byref(cus)))
+    return {"name": name.value.decode(), "wall_clock_khz": khz.value, "cus": cus.value}
+
+
+# ---------------------------------------------------------------------------
+# GPU context
+# ---------------------------------------------------------------------------
+class Context:
+    """One scheme instance on the GPU (table, index, CC state, transactions)."""
+
+    def __init__(self, cc_scheme: str, *, rows: int, cols: int = 10, warp_density: int = 5,
+                 block_size: int = 32, batch_size: int = 65536, seed: int = SEED,
+                 backoff_ticks: int = 20):
+        self._lib = load_library()
+        self.scheme = cc_scheme
+        self.rows, self.cols, self.seed = rows, cols, seed
+        cfg = Config(cc_scheme.encode(), warp_density, block_size, batch_size, rows, cols, seed,
+                     backoff_ticks, 0)
+        h = C.c_void_p()
+        _check(self._lib.gcctb_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.ntx = 0
+        self.nops = 0
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.gcctb_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def seed_table(self) -> None:
+        _check(self._lib.gcctb_seed_table(self._h))
+
+    def load_table(self, cells: np.ndarray) -> None:
+        cells = np.ascontiguousarray(cells, dtype=np.uint64).reshape(-1)
+        _check(self._lib.gcctb_load_table(self._h, cells, self.rows, self.cols))
+
+    def build_index(self, keys: np.ndarray, vals: np.ndarray) -> None:
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        vals = np.ascontiguousarray(vals, dtype=np.uint32)
+        _check(self._lib.gcctb_build_index(self._h, keys, vals, len(keys)))
+
+    def probe(self, keys: np.ndarray) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        out = np.zeros(max(1, len(keys)), dtype=np.uint32)
+        _check(self._lib.gcctb_probe(self._h, keys, len(keys), out))
+        return out[: len(keys)]
+
+    def load_ops(self, w: Ops) -> None:
+        ops = np.ascontiguousarray(w.ops, dtype=np.uint64)
+        if ops.size == 0:
+            ops = np.zeros(1, dtype=np.uint64)
+        _check(self._lib.gcctb_load_ops(self._h, ops, np.ascontiguousarray(w.tx_off, dtype=np.uint64), w.ntx))
+        self.ntx, self.nops = w.ntx, w.nops
+
+    def run_plain(self, check: bool = True) -> dict:
+        st = Stats()
+        rc = self._lib.gcctb_run_plain(self._h, C.byref(st))
+        if check:
+            _check(rc)
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    def fetch(self, table: bool = True, reads: bool = True, order: bool = True, aborts: bool = False):
+        t = np.zeros(self.rows * self.cols, dtype=np.uint64) if table else None
+        r = np.zeros(max(1, self.nops), dtype=np.uint64) if reads else None
+        o = np.zeros(max(1, 3 * self.ntx), dtype=np.uint64) if order else None
+        a = np.zeros(max(1, self.ntx), dtype=np.uint32) if aborts else None
+        _check(self._lib.gcctb_fetch(self._h, _ptr(t), _ptr(r), _ptr(o), _ptr(a)))
+        return {
+            "table": t,
+            "reads": None if r is None else r[: self.nops],
+            "order": None if o is None else o[: 3 * self.ntx],
+            "aborts": None if a is None else a[: self.ntx],
+        }
+
+    def fetch_ranks(self):
+        rank = np.zeros(max(1, self.ntx), dtype=np.uint32)
+        n = C.c_uint64(0)
+        _check(self._lib.gcctb_fetch_ranks(self._h, _ptr(rank), None, C.byref(n)))
+        counts = np.zeros(max(1, n.value), dtype=np.uint32)
+        _check(self._lib.gcctb_fetch_ranks(self._h, None, _ptr(counts), C.byref(n)))
+        return rank[: self.ntx], counts[: n.value]
+
+    def fetch_cc(self) -> np.ndarray:
+        w = np.zeros(self.rows, dtype=np.uint64)
+        _check(self._lib.gcctb_fetch_cc(self._h, w))
+        return w

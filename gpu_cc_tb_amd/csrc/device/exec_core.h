@@ -1,0 +1,911 @@
+// Transaction-execution kernel for gfx950 (the YCSB kernel of
+// benchmark/ycsb.cu:21-111, re-designed for wave64).
+//
+// Launch shape.  A transaction owns G = 2^warp_density consecutive lanes of a
+// wave (the reference gives each transaction 2^warp_density threads and runs it
+// on the first one, benchmark/ycsb.cu:32-38).  Here the G lanes cooperate: op j
+// of the transaction lives on lane (j mod G) as slot (j / G), so the ops of a
+// transaction are attempted in parallel and group decisions (abort, all locks
+// held, validation passed) are ballots over the group's lanes.
+//
+// Progress on CDNA.  A wave has a single program counter, so a lane that spins
+// on a word held by another lane of the same wave would never let that lane
+// run.  No lane ever spins: each op step returns OK / WAIT / ABORT, and the
+// wave loops in rounds over all of its groups until every transaction has
+// committed.  A waiting op is simply retried in the next round.
+//
+// Every transaction commits exactly once; the kernel records, per transaction,
+// the key of its serialization order (commit ticket, timestamp, or position)
+// and, per read op, the value it returned, so that a serial replay on the CPU
+// can reproduce the final table and every read.
+#pragma once
+#include "dev_util.h"
+
+namespace gcctb {
+namespace dev {
+
+// per-op slot flags
+constexpr uint32_t M_VALID = 1u;   // op exists and its key was found in the index
+constexpr uint32_t M_WRITE = 2u;
+constexpr uint32_t M_DONE = 4u;    // op finished in the current attempt
+constexpr uint32_t M_HELD = 8u;    // op holds a lock / uncommitted mark
+constexpr uint32_t M_COL_SHIFT = 8;
+
+// transaction phases
+constexpr uint32_t PH_START = 0;   // (re)start an attempt
+constexpr uint32_t PH_OPS = 1;     // per-op work (locks / reads / marks)
+constexpr uint32_t PH_WRITES = 2;  // MVCC: write marks after every read finished
+constexpr uint32_t PH_LOCK = 3;    // OCC: lock the write set
+constexpr uint32_t PH_COMMIT = 4;
+
+enum StepResult : uint32_t { R_OK = 0, R_WAIT = 1, R_ABORT = 2 };
+
+// Inline CAS retries before an op reports WAIT and yields to the next round.
+constexpr int CAS_TRIES = 8;
+
+template <int G>
+struct TxCtx {
+  static constexpr int K = (MAX_OPS + G - 1) / G;
+  uint64_t tx = 0;         // transaction id
+  uint64_t op0 = 0;        // index of op 0 in the CSR op array
+  uint32_t nops = 0;
+  uint32_t ph = PH_START;
+  uint32_t attempts = 0;   // aborted attempts
+  bool live = false;
+  bool done = true;
+  uint64_t ts = 0;         // attempt timestamp (T/O, MVCC) or commit timestamp
+  uint64_t t_start = 0, t_attempt = 0, t_resume = 0, t_round = 0;
+  uint64_t m_ts = 0, m_wait = 0, m_abort = 0, m_idx = 0, m_tot = 0, m_commit = 0;
+  uint32_t row[K];
+  uint32_t meta[K];
+  uint64_t st[K];
+};
+
+__device__ __forceinline__ uint64_t* cell_ptr(const ExecArgs& a, uint32_t row, uint32_t meta) {
+  return a.cells + (uint64_t)row * a.cols + (meta >> M_COL_SHIFT);
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t op_index(int k) {
+  return (lane_id() & (G - 1)) + (uint32_t)k * G;
+}
+
+// Abort bookkeeping, identical on every lane of the group.
+template <int G>
+__device__ __forceinline__ void abort_attempt(TxCtx<G>& c, const ExecArgs& a, uint64_t now) {
+  c.attempts++;
+  c.m_abort += now - c.t_attempt;
+  if (a.backoff_ticks) {
+    const uint64_t h = hash3(a.seed ^ TAG_BACKOFF, c.tx, c.attempts);
+    const uint32_t span = 1u << (c.attempts < 6 ? c.attempts : 6);
+    c.t_resume = now + (h % span) * a.backoff_ticks;
+  }
+  c.ph = PH_START;
+}
+
+template <int G>
+__device__ __forceinline__ void finish_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t now,
+                                          uint64_t key0, uint64_t key1) {
+  c.done = true;
+  c.m_tot += now - c.t_start;
+  c.m_commit += 1;
+  if ((lane_id() & (G - 1)) == 0) {
+    a.obatch[c.tx] = a.batch_id;
+    a.okey0[c.tx] = key0;
+    a.okey1[c.tx] = key1;
+    a.tx_aborts[c.tx] = c.attempts;
+  }
+}
+
+// Start a new attempt: clear per-op state.
+template <int G>
+__device__ __forceinline__ void begin_attempt(TxCtx<G>& c, uint64_t now) {
+  c.t_attempt = now;
+#pragma unroll
+  for (int k = 0; k < TxCtx<G>::K; ++k) c.meta[k] &= ~(M_DONE | M_HELD);
+}
+
+template <int G>
+__device__ __forceinline__ bool backoff_over(const TxCtx<G>& c, uint64_t now) {
+  return now >= c.t_resume;
+}
+
+// ===========================================================================
+// Two-phase locking, no-wait and wait-die (include/twopl.cuh:59-382).
+// Strict 2PL: shared locks for reads, exclusive locks for writes, writes are
+// applied at commit while every lock is held, locks released after.  The
+// commit ticket is drawn while all locks are held, so ticket order is a valid
+// serial order.
+// ===========================================================================
+template <bool WAIT_DIE>
+struct TwoPL {
+  // lock_shared (twopl.cuh:233-267)
+  static __device__ StepResult lock_shared(uint64_t* w, uint64_t self) {
+    uint64_t cur = ld(w);
+#pragma unroll 1
+    for (int t = 0; t < CAS_TRIES; ++t) {
+      uint64_t nxt;
+      if (tpl::cnt(cur) == 0) {
+        nxt = tpl::make(1, 1, tpl::holder(cur));
+      } else if (tpl::shared(cur)) {
+        nxt = cur + (1ull << 1);
+      } else {
+        const uint64_t h = tpl::holder(cur);
+        if (h == self) return R_OK;
+        if (!WAIT_DIE) return R_ABORT;
+        if (h < self) return R_ABORT;  // die: an older transaction holds it
+        return R_WAIT;                 // wait for a younger holder
+      }
+      if (cas(w, cur, nxt)) return R_OK;
+    }
+    return R_WAIT;
+  }
+  // lock_exclusive (twopl.cuh:269-303)
+  static __device__ StepResult lock_excl(uint64_t* w, uint64_t self) {
+    uint64_t cur = ld(w);
+#pragma unroll 1
+    for (int t = 0; t < CAS_TRIES; ++t) {
+      if (tpl::cnt(cur) == 0) {
+        if (cas(w, cur, tpl::make(0, 1, self))) return R_OK;
+        continue;
+      }
+      if (tpl::shared(cur)) return R_ABORT;
+      const uint64_t h = tpl::holder(cur);
+      if (h == self) return R_OK;
+      if (!WAIT_DIE) return R_ABORT;
+      if (h < self) return R_ABORT;
+      return R_WAIT;
+    }
+    return R_WAIT;
+  }
+  // unlock_shared: holder_cnt-- (twopl.cuh:305-315); unlock_exclusive clears
+  // holder_cnt (twopl.cuh:317-322) -- the word becomes free.
+  static __device__ void unlock(uint64_t* w, bool excl) {
+    if (excl)
+      st(w, 0);
+    else
+      fetch_sub(w, 1ull << 1);
+  }
+
+  template <int G>
+  static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
+    constexpr int K = TxCtx<G>::K;
+    const uint64_t now = now_ticks();
+    bool commit = false;
+    if (!c.done) {
+      if (c.ph == PH_START && backoff_over(c, now)) {
+        begin_attempt(c, now);
+        c.ph = PH_OPS;
+      }
+      if (c.ph == PH_OPS) {
+        bool conflict = false, pending = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const uint32_t m = c.meta[k];
+          if ((m & M_VALID) && !(m & M_HELD)) {
+            uint64_t* w = a.cc + c.row[k];
+            const StepResult r = (m & M_WRITE) ? lock_excl(w, c.tx) : lock_shared(w, c.tx);
+            if (r == R_OK)
+              c.meta[k] = m | M_HELD;
+            else if (r == R_ABORT)
+              conflict = true;
+            else
+              pending = true;
+          }
+        }
+        if (group_any<G>(conflict)) {
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (c.meta[k] & M_HELD) unlock(a.cc + c.row[k], c.meta[k] & M_WRITE);
+          abort_attempt(c, a, now);
+        } else if (group_all<G>(!pending)) {
+          c.ph = PH_COMMIT;
+        } else {
+          c.m_wait += now - c.t_round;
+        }
+      }
+      commit = (c.ph == PH_COMMIT);
+    }
+    uint64_t ticket = wave_fetch_inc(a.ctrs + 1, commit && (lane_id() & (G - 1)) == 0);
+    if (commit) {
+      ticket = group_bcast64<G>(ticket);
+      order_acquire();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint32_t m = c.meta[k];
+        if (!(m & M_VALID)) continue;
+        uint64_t* cell = cell_ptr(a, c.row[k], m);
+        const uint32_t j = op_index<G>(k);
+        if (m & M_WRITE)
+          st(cell, write_value(a.seed, c.tx, j));
+        else
+          a.reads[c.op0 + j] = ld(cell);
+      }
+      order_release();
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (c.meta[k] & M_VALID) unlock(a.cc + c.row[k], c.meta[k] & M_WRITE);
+      finish_tx(c, a, now, ticket, 0);
+    }
+  }
+};
+
+// ===========================================================================
+// Basic timestamp ordering (include/to.cuh:57-271).  Fresh timestamp per
+// attempt from the wave-aggregated allocator.  Read: abort if wts > ts, wait
+// while uncommitted, else read and raise rts by CAS.  Write: abort if rts > ts,
+// wts > ts or uncommitted, else mark {uncommitted, ts, ts}.  Commit installs
+// the data and clears the mark; abort restores the saved words.  The
+// serialization order is timestamp order.
+// ===========================================================================
+struct TimestampOrdering {
+  static __device__ StepResult read(uint64_t* w, uint64_t* cell, uint64_t ts, uint64_t& val) {
+    uint64_t cur = ld(w);
+#pragma unroll 1
+    for (int t = 0; t < CAS_TRIES; ++t) {
+      if (tsw::wts(cur) > ts) return R_ABORT;
+      if (tsw::unc(cur)) return R_WAIT;
+      order_acquire();
+      val = ld(cell);
+      order_acquire();
+      const uint64_t r = tsw::rts(cur) > ts ? tsw::rts(cur) : ts;
+      if (cas(w, cur, tsw::make(0, r, tsw::wts(cur)))) return R_OK;
+    }
+    return R_WAIT;
+  }
+  static __device__ StepResult write_mark(uint64_t* w, uint64_t ts, uint64_t& prev) {
+    uint64_t cur = ld(w);
+#pragma unroll 1
+    for (int t = 0; t < CAS_TRIES; ++t) {
+      if (tsw::rts(cur) > ts || tsw::wts(cur) > ts || tsw::unc(cur)) return R_ABORT;
+      prev = cur;
+      if (cas(w, cur, tsw::make(1, ts, ts))) return R_OK;
+    }
+    return R_WAIT;
+  }
+
+  template <int G>
+  static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
+    constexpr int K = TxCtx<G>::K;
+    const uint64_t now = now_ticks();
+    const bool start = !c.done && c.ph == PH_START && backoff_over(c, now);
+    uint64_t ts = wave_fetch_inc(a.ctrs + 0, start && (lane_id() & (G - 1)) == 0);
+    if (start) {
+      c.ts = group_bcast64<G>(ts);
+      c.m_ts += now_ticks() - now;
+      if (c.ts > tsw::TS_MAX) {
+        if ((lane_id() & (G - 1)) == 0) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_TS_OVERFLOW);
+        finish_tx(c, a, now, c.ts, 0);
+        return;
+      }
+      begin_attempt(c, now);
+      c.ph = PH_OPS;
+    }
+    if (c.done || c.ph != PH_OPS) return;
+    bool abort = false, pending = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t m = c.meta[k];
+      if (!(m & M_VALID) || (m & M_DONE)) continue;
+      uint64_t* w = a.cc + c.row[k];
+      if (m & M_WRITE) {
+        const StepResult r = write_mark(w, c.ts, c.st[k]);
+        if (r == R_OK)
+          c.meta[k] = m | M_DONE | M_HELD;
+        else if (r == R_ABORT)
+          abort = true;
+        else
+          pending = true;
+      } else {
+        uint64_t v;
+        const StepResult r = read(w, cell_ptr(a, c.row[k], m), c.ts, v);
+        if (r == R_OK) {
+          c.meta[k] = m | M_DONE;
+          a.reads[c.op0 + op_index<G>(k)] = v;
+        } else if (r == R_ABORT) {
+          abort = true;
+        } else {
+          pending = true;
+        }
+      }
+    }
+    if (group_any<G>(abort)) {
+      // rollback (to.cuh:255-270): restore the words this attempt marked
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+      abort_attempt(c, a, now);
+      return;
+    }
+    if (!group_all<G>(!pending)) {
+      c.m_wait += now - c.t_round;
+      return;
+    }
+    // commit (to.cuh:117-134)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD)
+        st(cell_ptr(a, c.row[k], c.meta[k]), write_value(a.seed, c.tx, op_index<G>(k)));
+    order_release();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], tsw::make(0, c.ts, c.ts));
+    finish_tx(c, a, now, c.ts, 0);
+  }
+};
+
+// ===========================================================================
+// Multi-version timestamp ordering (include/mvcc.cuh:61-297).
+// Per row: head word {uncommitted, rts, wts} + head of a version chain.  The
+// table always holds the newest committed version; a commit pushes the old
+// value of the written column into a node of the version pool (device bump
+// allocator).  A read at ts returns the newest version with wts <= ts and never
+// aborts; it waits while the head is uncommitted (mvcc.cuh:169-203).  Writes
+// abort like T/O (mvcc.cuh:240-241).  Reads run before write marks so that an
+// attempt never waits while it holds an uncommitted mark.
+// Node layout (3 x u64): [0] ts word of the superseded version,
+// [1] old value of the written column, [2] previous node + 1 | column << 32.
+// ===========================================================================
+struct Mvcc {
+  static __device__ StepResult read(const ExecArgs& a, uint32_t row, uint32_t col, uint64_t ts,
+                                    uint64_t& val) {
+    uint64_t* w = a.cc + row;
+    uint64_t* cell = a.cells + (uint64_t)row * a.cols + col;
+#pragma unroll 1
+    for (int t = 0; t < CAS_TRIES; ++t) {
+      uint64_t cur = ld(w);
+      if (tsw::unc(cur)) return R_WAIT;
+      if (tsw::wts(cur) <= ts) {
+        order_acquire();
+        val = ld(cell);
+        order_acquire();
+        const uint64_t r = tsw::rts(cur) > ts ? tsw::rts(cur) : ts;
+        if (cas(w, cur, tsw::make(0, r, tsw::wts(cur)))) return R_OK;
+        continue;
+      }
+      // walk back to the newest version with wts <= ts
+      order_acquire();
+      uint64_t v = ld(cell);
+      uint32_t p = ld32(a.vprev + row);
+      uint64_t cur_wts = tsw::wts(cur);
+      bool broken = false;
+#pragma unroll 1
+      while (cur_wts > ts) {
+        if (p == 0) {
+          broken = true;
+          break;
+        }
+        const uint64_t* node = a.pool + (uint64_t)(p - 1) * 3;
+        const uint64_t link = ld(node + 2);
+        if ((uint32_t)(link >> 32) == col) v = ld(node + 1);
+        cur_wts = tsw::wts(ld(node + 0));
+        p = (uint32_t)link;
+      }
+      order_acquire();
+      if (ld(w) != cur) continue;  // the head moved: retry (mvcc.cuh:196)
+      if (broken) {
+        atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_CHAIN_BROKEN);
+        val = 0;
+        return R_OK;
+      }
+      val = v;
+      return R_OK;
+    }
+    return R_WAIT;
+  }
+
+  template <int G>
+  static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
+    constexpr int K = TxCtx<G>::K;
+    const uint64_t now = now_ticks();
+    const bool start = !c.done && c.ph == PH_START && backoff_over(c, now);
+    uint64_t ts = wave_fetch_inc(a.ctrs + 0, start && (lane_id() & (G - 1)) == 0);
+    if (start) {
+      c.ts = group_bcast64<G>(ts);
+      c.m_ts += now_ticks() - now;
+      if (c.ts > tsw::TS_MAX) {
+        if ((lane_id() & (G - 1)) == 0) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_TS_OVERFLOW);
+        finish_tx(c, a, now, c.ts, 0);
+      } else {
+        begin_attempt(c, now);
+        c.ph = PH_OPS;
+      }
+    }
+    bool commit = false;
+    if (!c.done && c.ph == PH_OPS) {
+      bool pending = false;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint32_t m = c.meta[k];
+        if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
+        uint64_t v;
+        if (read(a, c.row[k], m >> M_COL_SHIFT, c.ts, v) == R_OK) {
+          c.meta[k] = m | M_DONE;
+          a.reads[c.op0 + op_index<G>(k)] = v;
+        } else {
+          pending = true;
+        }
+      }
+      if (group_all<G>(!pending))
+        c.ph = PH_WRITES;
+      else
+        c.m_wait += now - c.t_round;
+    }
+    if (!c.done && c.ph == PH_WRITES) {
+      bool abort = false, pending = false;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint32_t m = c.meta[k];
+        if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_DONE)) continue;
+        uint64_t* w = a.cc + c.row[k];
+        uint64_t cur = ld(w);
+        StepResult r = R_WAIT;
+#pragma unroll 1
+        for (int t = 0; t < CAS_TRIES; ++t) {
+          if (tsw::rts(cur) > c.ts || tsw::wts(cur) > c.ts || tsw::unc(cur)) {
+            r = R_ABORT;
+            break;
+          }
+          c.st[k] = cur;
+          if (cas(w, cur, tsw::make(1, c.ts, c.ts))) {
+            r = R_OK;
+            break;
+          }
+        }
+        if (r == R_OK)
+          c.meta[k] = m | M_DONE | M_HELD;
+        else if (r == R_ABORT)
+          abort = true;
+        else
+          pending = true;
+      }
+      if (group_any<G>(abort)) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+        abort_attempt(c, a, now);
+      } else if (group_all<G>(!pending)) {
+        commit = true;
+      }
+    }
+    // version-node allocation: one wave-wide atomic
+    uint32_t nw = 0;
+    if (commit) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) nw += (c.meta[k] & M_HELD) ? 1u : 0u;
+    }
+    uint64_t node = wave_fetch_add(a.ctrs + 2, nw);
+    if (!commit) return;
+    if (node + nw > a.pool_cap) {
+      atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_POOL_OVERFLOW);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t m = c.meta[k];
+      if (!(m & M_HELD)) continue;
+      const uint32_t row = c.row[k];
+      const uint32_t col = m >> M_COL_SHIFT;
+      uint64_t* cell = a.cells + (uint64_t)row * a.cols + col;
+      if (node < a.pool_cap) {
+        uint64_t* nd = a.pool + node * 3;
+        const uint64_t old = ld(cell);
+        const uint32_t prev = ld32(a.vprev + row);
+        st(nd + 0, c.st[k]);
+        st(nd + 1, old);
+        st(nd + 2, (uint64_t)prev | ((uint64_t)col << 32));
+        st(cell, write_value(a.seed, c.tx, op_index<G>(k)));
+        order_release();
+        st32(a.vprev + row, (uint32_t)(node + 1));
+      } else {
+        st(cell, write_value(a.seed, c.tx, op_index<G>(k)));
+      }
+      node++;
+    }
+    order_release();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], tsw::make(0, c.ts, c.ts));
+    finish_tx(c, a, now, c.ts, 0);
+  }
+};
+
+// ===========================================================================
+// Silo OCC (include/silo.cuh:70-339).
+// Read: consistent snapshot (word, data, word unchanged and unlocked).
+// Validation: lock the write set (no-wait), draw the commit ticket, compute
+// commit_ts = max(read ts, current ts of the locked write words) + 1, abort if a
+// read word changed or is locked; write phase installs data and publishes
+// {unlocked, commit_ts}.  The ticket drawn between locking and read validation
+// is the serialization order.
+// ===========================================================================
+struct Silo {
+  template <int G>
+  static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
+    constexpr int K = TxCtx<G>::K;
+    const uint64_t now = now_ticks();
+    bool validate = false;
+    if (!c.done) {
+      if (c.ph == PH_START && backoff_over(c, now)) {
+        begin_attempt(c, now);
+        c.ph = PH_OPS;
+      }
+      if (c.ph == PH_OPS) {
+        bool pending = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const uint32_t m = c.meta[k];
+          if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
+          uint64_t* w = a.cc + c.row[k];
+          const uint64_t w1 = ld(w);
+          order_acquire();
+          const uint64_t v = ld(cell_ptr(a, c.row[k], m));
+          order_acquire();
+          const uint64_t w2 = ld(w);
+          if (w1 != w2 || silo::locked(w1)) {
+            pending = true;
+            continue;
+          }
+          c.st[k] = silo::ts(w1);
+          c.meta[k] = m | M_DONE;
+          a.reads[c.op0 + op_index<G>(k)] = v;
+        }
+        if (group_all<G>(!pending))
+          c.ph = PH_LOCK;
+        else
+          c.m_wait += now - c.t_round;
+      }
+      if (c.ph == PH_LOCK) {
+        bool conflict = false, pending = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const uint32_t m = c.meta[k];
+          if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_HELD)) continue;
+          uint64_t* w = a.cc + c.row[k];
+          uint64_t cur = ld(w);
+          StepResult r = R_WAIT;
+#pragma unroll 1
+          for (int t = 0; t < CAS_TRIES; ++t) {
+            if (silo::locked(cur)) {
+              r = R_ABORT;
+              break;
+            }
+            if (cas(w, cur, cur | 1ull)) {
+              r = R_OK;
+              break;
+            }
+          }
+          if (r == R_OK) {
+            c.st[k] = silo::ts(cur);
+            c.meta[k] = m | M_HELD;
+          } else if (r == R_ABORT) {
+            conflict = true;
+          } else {
+            pending = true;
+          }
+        }
+        if (group_any<G>(conflict)) {
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (c.meta[k] & M_HELD) st(a.cc + c.row[k], silo::make(0, c.st[k]));
+          abort_attempt(c, a, now);
+        } else if (group_all<G>(!pending)) {
+          validate = true;
+        }
+      }
+    }
+    order_acquire();
+    uint64_t ticket = wave_fetch_inc(a.ctrs + 1, validate && (lane_id() & (G - 1)) == 0);
+    if (!validate) return;
+    ticket = group_bcast64<G>(ticket);
+    order_acquire();
+    uint64_t mx = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((c.meta[k] & M_VALID) && c.st[k] > mx) mx = c.st[k];
+    const uint64_t commit_ts = group_max64<G>(mx) + 1;
+    bool fail = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t m = c.meta[k];
+      if (!(m & M_VALID) || (m & M_WRITE)) continue;
+      const uint64_t w = ld(a.cc + c.row[k]);
+      if (silo::ts(w) != c.st[k] || silo::locked(w)) fail = true;
+    }
+    if (group_any<G>(fail)) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (c.meta[k] & M_HELD) st(a.cc + c.row[k], silo::make(0, c.st[k]));
+      abort_attempt(c, a, now);
+      return;
+    }
+    // write phase (silo.cuh:323-338)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD)
+        st(cell_ptr(a, c.row[k], c.meta[k]), write_value(a.seed, c.tx, op_index<G>(k)));
+    order_release();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], silo::make(0, commit_ts));
+    finish_tx(c, a, now, ticket, 0);
+  }
+};
+
+// ===========================================================================
+// TicToc OCC (include/tictoc.cuh:67-369).
+// Read: snapshot {wts, rts = wts + delta}.  Validation: lock the write set
+// (no-wait); commit_ts = max(max over writes of rts + 1, max over reads of
+// wts); each read whose rts < commit_ts is re-checked (abort if wts changed or
+// it is locked by another) and its rts extended to commit_ts by CAS, shifting
+// wts when delta overflows 15 bits (tictoc.cuh:302-341).  The commit ticket is
+// drawn after validation while the write locks are held; (commit_ts, ticket)
+// is the serialization order.
+// ===========================================================================
+struct Tictoc {
+  template <int G>
+  static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
+    constexpr int K = TxCtx<G>::K;
+    const uint64_t now = now_ticks();
+    bool commit = false;
+    uint64_t commit_ts = 0;
+    if (!c.done) {
+      if (c.ph == PH_START && backoff_over(c, now)) {
+        begin_attempt(c, now);
+        c.ph = PH_OPS;
+      }
+      if (c.ph == PH_OPS) {
+        bool pending = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const uint32_t m = c.meta[k];
+          if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
+          uint64_t* w = a.cc + c.row[k];
+          const uint64_t w1 = ld(w);
+          order_acquire();
+          const uint64_t v = ld(cell_ptr(a, c.row[k], m));
+          order_acquire();
+          const uint64_t w2 = ld(w);
+          if (w1 != w2 || tictoc::locked(w1)) {
+            pending = true;
+            continue;
+          }
+          c.st[k] = w1;
+          c.meta[k] = m | M_DONE;
+          a.reads[c.op0 + op_index<G>(k)] = v;
+        }
+        if (group_all<G>(!pending))
+          c.ph = PH_LOCK;
+        else
+          c.m_wait += now - c.t_round;
+      }
+      if (c.ph == PH_LOCK) {
+        bool conflict = false, pending = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const uint32_t m = c.meta[k];
+          if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_HELD)) continue;
+          uint64_t* w = a.cc + c.row[k];
+          uint64_t cur = ld(w);
+          StepResult r = R_WAIT;
+#pragma unroll 1
+          for (int t = 0; t < CAS_TRIES; ++t) {
+            if (tictoc::locked(cur)) {
+              r = R_ABORT;
+              break;
+            }
+            if (cas(w, cur, cur | 1ull)) {
+              r = R_OK;
+              break;
+            }
+          }
+          if (r == R_OK) {
+            c.st[k] = cur;
+            c.meta[k] = m | M_HELD;
+          } else if (r == R_ABORT) {
+            conflict = true;
+          } else {
+            pending = true;
+          }
+        }
+        if (group_any<G>(conflict)) {
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+          abort_attempt(c, a, now);
+        } else if (group_all<G>(!pending)) {
+          // validation (tictoc.cuh:262-350)
+          order_acquire();
+          uint64_t mx = 0;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const uint32_t m = c.meta[k];
+            if (!(m & M_VALID)) continue;
+            const uint64_t cand = (m & M_WRITE) ? tictoc::rts(c.st[k]) + 1 : tictoc::wts(c.st[k]);
+            if (cand > mx) mx = cand;
+          }
+          commit_ts = group_max64<G>(mx);
+          bool fail = false;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const uint32_t m = c.meta[k];
+            if (!(m & M_VALID) || (m & M_WRITE)) continue;
+            if (tictoc::rts(c.st[k]) >= commit_ts) continue;
+            uint64_t* w = a.cc + c.row[k];
+            uint64_t cur = ld(w);
+#pragma unroll 1
+            while (true) {
+              const uint64_t wts = tictoc::wts(cur);
+              const uint64_t rts = tictoc::rts(cur);
+              if (wts != tictoc::wts(c.st[k]) || (rts <= commit_ts && tictoc::locked(cur))) {
+                fail = true;
+                break;
+              }
+              if (rts > commit_ts) break;
+              const uint64_t delta = commit_ts - wts;
+              const uint64_t shift = delta - (delta & 0x7fffull);
+              const uint64_t nxt = tictoc::make(tictoc::locked(cur), delta - shift, wts + shift);
+              if (cas(w, cur, nxt)) break;
+            }
+          }
+          if (group_any<G>(fail)) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+              if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+            abort_attempt(c, a, now);
+          } else {
+            commit = true;
+          }
+        }
+      }
+    }
+    order_acquire();
+    uint64_t ticket = wave_fetch_inc(a.ctrs + 1, commit && (lane_id() & (G - 1)) == 0);
+    if (!commit) return;
+    ticket = group_bcast64<G>(ticket);
+    // write phase (tictoc.cuh:352-368)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD)
+        st(cell_ptr(a, c.row[k], c.meta[k]), write_value(a.seed, c.tx, op_index<G>(k)));
+    order_release();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], tictoc::make(0, 0, commit_ts));
+    finish_tx(c, a, now, commit_ts, ticket);
+  }
+};
+
+// ===========================================================================
+// Ordered execution for GaccO (include/gacco.cuh:49-176) and GPUTx
+// (include/gputx.cuh:58-146).  Preprocessing sorts every access of the batch
+// by (row, order key) and gives op j its access group start need[j] (a write
+// is its own group; consecutive reads share one).  counters[row] starts at the
+// row's first position and counts completed accesses; an op runs once
+// counters[row] >= need, then increments it.  Each row therefore sees its
+// accesses in order-key order -- ascending transaction id for GaccO, (rank,
+// transaction id) for GPUTx -- which makes the outcome that of the serial
+// execution in that order.  No aborts.
+// ===========================================================================
+struct Ordered {
+  template <int G>
+  static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
+    constexpr int K = TxCtx<G>::K;
+    if (c.done) return;
+    const uint64_t now = now_ticks();
+    if (c.ph == PH_START) {
+      begin_attempt(c, now);
+      c.ph = PH_OPS;
+    }
+    bool pending = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t m = c.meta[k];
+      if (!(m & M_VALID) || (m & M_DONE)) continue;
+      const uint32_t j = op_index<G>(k);
+      const uint32_t need = a.need[c.op0 + j];
+      uint32_t* ctr = a.counters + c.row[k];
+      if (ld32(ctr) < need) {
+        pending = true;
+        continue;
+      }
+      order_acquire();
+      uint64_t* cell = cell_ptr(a, c.row[k], m);
+      if (m & M_WRITE)
+        st(cell, write_value(a.seed, c.tx, j));
+      else
+        a.reads[c.op0 + j] = ld(cell);
+      order_release();
+      fetch_add32(ctr, 1u);
+      c.meta[k] = m | M_DONE;
+    }
+    if (group_all<G>(!pending)) {
+      const uint64_t key0 = a.tx_rank ? a.tx_rank[c.tx] : c.tx;
+      finish_tx(c, a, now, key0, c.tx);
+    } else {
+      c.m_wait += now - c.t_round;
+    }
+  }
+};
+
+// ===========================================================================
+// The kernel.
+// ===========================================================================
+template <class S, int G>
+__global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
+  constexpr int K = TxCtx<G>::K;
+  TxCtx<G> c;
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t slot = a.slot_lo + gtid / G;
+  c.live = slot < a.slot_hi;
+  c.done = !c.live;
+  if (c.live) {
+    c.tx = a.tx_map ? a.tx_map[slot] : slot;
+    c.op0 = a.tx_off[c.tx];
+    c.nops = (uint32_t)(a.tx_off[c.tx + 1] - c.op0);
+  }
+  c.t_start = now_ticks();
+  c.t_round = c.t_start;
+  uint64_t misses = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t j = op_index<G>(k);
+    c.meta[k] = 0;
+    c.row[k] = 0;
+    c.st[k] = 0;
+    if (c.live && j < c.nops) {
+      const uint64_t op = a.ops[c.op0 + j];
+      const uint32_t row = index_find(a.index, op_key(op));
+      a.rows_out[c.op0 + j] = row;
+      if (row < a.rows) {
+        c.row[k] = row;
+        c.meta[k] = M_VALID | (op_is_write(op) ? M_WRITE : 0u) | (op_col(op) << M_COL_SHIFT);
+      } else {
+        misses++;
+      }
+    }
+  }
+  if (c.live && c.nops > (uint32_t)(K * G)) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_TXLEN);
+  c.m_idx = now_ticks() - c.t_start;
+#pragma unroll 1
+  while (ballot(!c.done) != 0) {
+    S::round(c, a);
+    c.t_round = now_ticks();
+    // Watchdog: the wall clock is a scalar read, so the exit is wave-uniform.
+    if (a.watchdog_ticks && c.t_round - c.t_start > a.watchdog_ticks) {
+      if (lane_id() == 0) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_WATCHDOG);
+      break;
+    }
+  }
+  // metrics: one value per transaction (group leader), reduced over the wave
+  const bool leader = c.live && (lane_id() & (G - 1)) == 0;
+  uint64_t v[8] = {leader ? c.attempts : 0ull, leader ? c.m_ts : 0ull, leader ? c.m_wait : 0ull,
+                   leader ? c.m_abort : 0ull, leader ? (c.m_tot - c.m_idx) : 0ull,
+                   leader ? c.m_tot : 0ull,   leader ? c.m_idx : 0ull, leader ? c.m_commit : 0ull};
+  misses = group_sum64<G>(misses);
+  const uint64_t miss_leader = leader ? misses : 0ull;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    uint64_t x = v[f];
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x += shfl_xor64(x, m);
+    v[f] = x;
+  }
+  uint64_t mm = miss_leader;
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) mm += shfl_xor64(mm, m);
+  if (lane_id() == 0) {
+    unsigned long long* M = (unsigned long long*)a.metrics;
+    if (v[0]) atomicAdd(M + 0, v[0]);
+    atomicAdd(M + 1, v[1]);
+    atomicAdd(M + 2, v[2]);
+    atomicAdd(M + 3, v[3]);
+    atomicAdd(M + 4, v[4]);
+    atomicAdd(M + 5, v[5]);
+    atomicAdd(M + 6, v[6]);
+    atomicAdd(M + 7, v[7]);
+    if (mm) atomicAdd(M + 8, mm);
+  }
+}
+
+}  // namespace dev
+}  // namespace gcctb

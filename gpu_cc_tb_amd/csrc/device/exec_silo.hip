@@ -1,0 +1,9 @@
+// Execution kernels for Silo (include/silo.cuh).
+#include "exec_launch.h"
+namespace gcctb {
+namespace launch {
+hipError_t exec_silo(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s) {
+  return dev::launch_exec_g<dev::Silo>(G, a, nslots, threads, s);
+}
+}  // namespace launch
+}  // namespace gcctb

@@ -1,0 +1,86 @@
+// Host-side launch wrappers for the gfx950 kernels.  Every wrapper enqueues on
+// the given stream and returns hipSuccess or the launch error.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../gcctb_common.h"
+
+namespace gcctb {
+namespace launch {
+
+// --- execution kernels (one translation unit per scheme family) ---
+// G must be 1, 2, 4, 8, 16 or 32; threads = block threads.
+hipError_t exec_tpl(bool wait_die, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_to(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_mvcc(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_silo(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_tictoc(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_ordered(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+
+// --- table / index ---
+hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s);
+// leaves from sorted keys/vals (device arrays); keys==nullptr: identity (key i -> value i)
+hipError_t build_leaves(uint32_t* leaves, const uint32_t* keys, const uint32_t* vals, uint64_t n,
+                        uint64_t nleaves, hipStream_t s);
+hipError_t build_level_from_leaves(uint32_t* dst, const uint32_t* leaves, uint64_t nleaves,
+                                   uint64_t padded, hipStream_t s);
+hipError_t build_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t nsrc_nodes,
+                                  uint64_t padded, hipStream_t s);
+hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
+                 hipStream_t s);
+
+// --- per-op helpers ---
+hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s);
+hipError_t iota_u32(uint32_t* v, uint64_t n, uint32_t base, hipStream_t s);
+hipError_t fill_u32(uint32_t* v, uint64_t n, uint32_t x, hipStream_t s);
+hipError_t fill_u64(uint64_t* v, uint64_t n, uint64_t x, hipStream_t s);
+
+// --- stable LSD radix sort (8-bit digits) of (key, value) pairs ---
+// Sorts bits [0, key_bits) of the keys.  Buffers: keys/vals in, alt buffers of
+// the same size; the sorted result ends in (*keys_out, *vals_out), which point
+// to one of the two buffer pairs.  tmp must hold radix_sort_tmp_words(n) u32.
+uint64_t radix_sort_tmp_words(uint64_t n);
+hipError_t radix_sort_u32(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint64_t n,
+                          int key_bits, uint32_t* tmp, uint32_t** keys_out, uint32_t** vals_out,
+                          hipStream_t s);
+hipError_t radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t n,
+                          int key_bits, uint32_t* tmp, uint64_t** keys_out, uint32_t** vals_out,
+                          hipStream_t s);
+
+// --- scans (u32) --- tmp must hold scan_tmp_words(n) u32.
+uint64_t scan_tmp_words(uint64_t n);
+hipError_t exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
+hipError_t inclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
+hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
+
+// --- ordered-execution preprocessing ---
+// From accesses sorted by (row, order): group-start flags (p or 0).
+hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
+                              uint64_t n, uint32_t* flags, hipStream_t s);
+// need[sop[p]] = gstart[p] + pos_base; counters[srow[p]] = p + pos_base at row starts.
+// same_order_conflicts counts adjacent same-key conflicting pairs (okey may be null).
+hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
+                       uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
+                       uint64_t nrows, const uint32_t* op_tx, const uint32_t* tx_rank,
+                       const uint64_t* ops, uint64_t* same_order_conflicts, hipStream_t s);
+// GPUTx one-pass rank (src/gputx.cu:5-27).  boundary[p] = 1 if p is not the
+// first access of its row and (p-1 or p) is a write; rowstart[p] = p at row
+// starts else 0.
+hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
+                     uint64_t nrows, uint32_t* boundary, uint32_t* rowstart, hipStream_t s);
+// local rank = S[p] - S[SS[p]]; rank[tx] = max; max_rank = max.
+hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                    const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s);
+// composite key (row << rank_bits) | rank[op_tx[op]] for ops [0, n)
+hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                         uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s);
+hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s);
+hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t* hist, hipStream_t s);
+
+// --- per-batch state ---
+hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
+                      uint32_t* vprev, hipStream_t s);
+hipError_t batch_begin(uint64_t* ctrs, hipStream_t s);
+
+}  // namespace launch
+}  // namespace gcctb

@@ -1,0 +1,517 @@
+// Preprocessing and maintenance kernels for gfx950: table seeding, sorted-array
+// index construction and probing, a stable LSD radix sort, scans, and the
+// GaccO / GPUTx access-order passes that replace the reference's Thrust calls
+// (include/gacco.cuh:307-336, include/gputx.cuh:252-281).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "dev_util.h"
+#include "launch.h"
+
+namespace gcctb {
+namespace {
+
+constexpr int BT = 256;  // threads per block for the streaming kernels
+
+inline unsigned grid_for(uint64_t n, int per_block) {
+  uint64_t g = (n + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  if (g > 0x7fffffffull) g = 0x7fffffffull;
+  return (unsigned)g;
+}
+inline unsigned grid_stride_blocks(uint64_t n) {
+  uint64_t g = (n + BT - 1) / BT;
+  if (g > 65536 * 4) g = 65536 * 4;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+__global__ void k_seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed) {
+  const uint64_t n = rows * cols;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = i / cols;
+    cells[i] = cell_init(seed, r, (uint32_t)(i - r * cols), cols);
+  }
+}
+
+// leaf l holds keys[8l..8l+7] then vals[8l..8l+7]; missing entries get key
+// 0xffffffff (never a valid key) and value 0.
+__global__ void k_build_leaves(uint32_t* leaves, const uint32_t* keys, const uint32_t* vals,
+                               uint64_t n, uint64_t nleaves) {
+  const uint64_t total = nleaves * IDX_LEAF;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t l = i / IDX_LEAF, e = i % IDX_LEAF;
+    uint32_t k = 0xffffffffu, v = 0;
+    if (i < n) {
+      k = keys ? keys[i] : (uint32_t)i;
+      v = vals ? vals[i] : (uint32_t)i;
+    }
+    leaves[l * 2 * IDX_LEAF + e] = k;
+    leaves[l * 2 * IDX_LEAF + IDX_LEAF + e] = v;
+  }
+}
+__global__ void k_level_from_leaves(uint32_t* dst, const uint32_t* leaves, uint64_t nleaves,
+                                    uint64_t padded) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < padded;
+       j += (uint64_t)gridDim.x * blockDim.x)
+    dst[j] = j < nleaves ? leaves[j * 2 * IDX_LEAF] : 0xffffffffu;
+}
+__global__ void k_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t nsrc_nodes,
+                                   uint64_t padded) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < padded;
+       j += (uint64_t)gridDim.x * blockDim.x)
+    dst[j] = j < nsrc_nodes ? src[j * IDX_FANOUT] : 0xffffffffu;
+}
+
+__global__ void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    rows_out[i] = dev::index_find(ix, op_key(ops[i]));
+}
+
+__global__ void k_op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ntx;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = tx_off[t], e = tx_off[t + 1];
+    for (uint64_t i = b; i < e; ++i) op_tx[i] = (uint32_t)t;
+  }
+}
+__global__ void k_iota(uint32_t* v, uint64_t n, uint32_t base) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    v[i] = base + (uint32_t)i;
+}
+__global__ void k_fill32(uint32_t* v, uint64_t n, uint32_t x) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    v[i] = x;
+}
+__global__ void k_fill64(uint64_t* v, uint64_t n, uint64_t x) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    v[i] = x;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort.  A tile is RS_ITEMS x RS_BT elements; element e of
+// the tile is handled in iteration e / RS_BT by thread e % RS_BT, so
+// (iteration, thread) order is input order and the in-tile rank is stable.
+// Within a wave, lanes with equal digits are found with 8 ballots; waves are
+// combined through per-digit LDS counters.
+// ---------------------------------------------------------------------------
+constexpr int RS_BT = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_BT * RS_ITEMS;
+constexpr int RS_WAVES = RS_BT / 64;
+
+template <class KT>
+__global__ __launch_bounds__(RS_BT) void k_rs_hist(const KT* keys, uint64_t n, int shift,
+                                                   uint32_t* hist, uint32_t nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int i = 0; i < RS_ITEMS; ++i) {
+    const uint64_t e = base + (uint64_t)i * RS_BT + threadIdx.x;
+    if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+template <class KT>
+__global__ __launch_bounds__(RS_BT) void k_rs_scatter(const KT* kin, const uint32_t* vin, KT* kout,
+                                                      uint32_t* vout, uint64_t n, int shift,
+                                                      const uint32_t* offs, uint32_t nb) {
+  __shared__ uint32_t gofs[256];
+  __shared__ uint32_t wcnt[RS_WAVES][256];
+  gofs[threadIdx.x] = offs[(uint64_t)threadIdx.x * nb + blockIdx.x];
+  const uint32_t lane = dev::lane_id();
+  const uint32_t wave = threadIdx.x / 64;
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int i = 0; i < RS_ITEMS; ++i) {
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t e = base + (uint64_t)i * RS_BT + threadIdx.x;
+    const bool valid = e < n;
+    KT key = 0;
+    uint32_t val = 0, d = 0;
+    if (valid) {
+      key = kin[e];
+      val = vin[e];
+      d = (uint32_t)(key >> shift) & 0xffu;
+    }
+    uint64_t peers = dev::ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = dev::ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t below = (uint32_t)__popcll(peers & dev::lanemask_lt());
+    if (valid && below == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      // exclusive prefix over waves for digit t, then advance the tile base
+      const uint32_t t = threadIdx.x;
+      uint32_t run = gofs[t];
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) {
+        const uint32_t c = wcnt[w][t];
+        wcnt[w][t] = run;
+        run += c;
+      }
+      gofs[t] = run;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = wcnt[wave][d] + below;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scans over u32 with + or max.  Three kernels: block totals, a single-block
+// scan of the totals, and the block-local scan with its carry-in.
+// ---------------------------------------------------------------------------
+constexpr int SC_BT = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_BT * SC_ITEMS;
+
+struct OpAdd {
+  static __device__ __forceinline__ uint32_t id() { return 0u; }
+  static __device__ __forceinline__ uint32_t f(uint32_t a, uint32_t b) { return a + b; }
+};
+struct OpMax {
+  static __device__ __forceinline__ uint32_t id() { return 0u; }
+  static __device__ __forceinline__ uint32_t f(uint32_t a, uint32_t b) { return a > b ? a : b; }
+};
+
+// inclusive scan across the block of one value per thread; returns inclusive,
+// writes the block total to *total.
+template <class Op>
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[SC_BT / 64];
+  const uint32_t lane = dev::lane_id(), wave = threadIdx.x / 64;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)x, d, 64);
+    if (lane >= (uint32_t)d) x = Op::f(o, x);
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  uint32_t carry = Op::id();
+  for (uint32_t w = 0; w < wave; ++w) carry = Op::f(carry, wsum[w]);
+  uint32_t t = Op::id();
+  for (int w = 0; w < SC_BT / 64; ++w) t = Op::f(t, wsum[w]);
+  *total = t;
+  __syncthreads();
+  return Op::f(carry, x);
+}
+
+template <class Op>
+__global__ __launch_bounds__(SC_BT) void k_scan_reduce(const uint32_t* in, uint64_t n, uint32_t* sums) {
+  const uint64_t base = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
+  uint32_t acc = Op::id();
+  for (int i = 0; i < SC_ITEMS; ++i)
+    if (base + i < n) acc = Op::f(acc, in[base + i]);
+  uint32_t total;
+  block_scan<Op>(acc, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+// exclusive scan of sums[0..nb) in place (single block)
+template <class Op>
+__global__ __launch_bounds__(SC_BT) void k_scan_top(uint32_t* sums, uint32_t nb) {
+  __shared__ uint32_t tincl[SC_BT];
+  const uint32_t per = (nb + SC_BT - 1) / SC_BT;
+  const uint32_t b = threadIdx.x * per;
+  uint32_t acc = Op::id();
+  for (uint32_t i = 0; i < per; ++i)
+    if (b + i < nb) acc = Op::f(acc, sums[b + i]);
+  uint32_t total;
+  const uint32_t incl = block_scan<Op>(acc, &total);
+  tincl[threadIdx.x] = incl;
+  __syncthreads();
+  uint32_t c = threadIdx.x == 0 ? Op::id() : tincl[threadIdx.x - 1];
+  for (uint32_t i = 0; i < per; ++i) {
+    if (b + i < nb) {
+      const uint32_t v = sums[b + i];
+      sums[b + i] = c;
+      c = Op::f(c, v);
+    }
+  }
+}
+template <class Op, bool INCLUSIVE>
+__global__ __launch_bounds__(SC_BT) void k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                     const uint32_t* sums) {
+  const uint64_t base = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
+  uint32_t vals[SC_ITEMS];
+  uint32_t acc = Op::id();
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    vals[i] = (base + i < n) ? in[base + i] : Op::id();
+    acc = Op::f(acc, vals[i]);
+  }
+  uint32_t total;
+  const uint32_t incl = block_scan<Op>(acc, &total);
+  // exclusive prefix for this thread = block carry + (incl of previous threads)
+  __shared__ uint32_t tincl[SC_BT];
+  tincl[threadIdx.x] = incl;
+  __syncthreads();
+  uint32_t c = Op::f(sums[blockIdx.x], threadIdx.x == 0 ? Op::id() : tincl[threadIdx.x - 1]);
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    if (base + i < n) {
+      if (INCLUSIVE) {
+        c = Op::f(c, vals[i]);
+        out[base + i] = c;
+      } else {
+        out[base + i] = c;
+        c = Op::f(c, vals[i]);
+      }
+    }
+  }
+}
+
+template <class Op, bool INCLUSIVE>
+hipError_t scan_impl(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t nb = (n + SC_TILE - 1) / SC_TILE;
+  hipLaunchKernelGGL((k_scan_reduce<Op>), dim3((unsigned)nb), dim3(SC_BT), 0, s, in, n, tmp);
+  hipLaunchKernelGGL((k_scan_top<Op>), dim3(1), dim3(SC_BT), 0, s, tmp, (uint32_t)nb);
+  hipLaunchKernelGGL((k_scan_down<Op, INCLUSIVE>), dim3((unsigned)nb), dim3(SC_BT), 0, s, in, out, n,
+                     tmp);
+  return hipGetLastError();
+}
+
+template <class KT>
+hipError_t radix_impl(KT* k0, uint32_t* v0, KT* k1, uint32_t* v1, uint64_t n, int key_bits,
+                      uint32_t* tmp, KT** kout, uint32_t** vout, hipStream_t s) {
+  KT* ka = k0;
+  uint32_t* va = v0;
+  KT* kb = k1;
+  uint32_t* vb = v1;
+  if (n > 1 && key_bits > 0) {
+    const uint64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    uint32_t* hist = tmp;
+    uint32_t* stmp = tmp + 256 * nb;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+      hipLaunchKernelGGL((k_rs_hist<KT>), dim3((unsigned)nb), dim3(RS_BT), 0, s, ka, n, shift, hist,
+                         (uint32_t)nb);
+      hipError_t e = scan_impl<OpAdd, false>(hist, hist, 256 * nb, stmp, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_rs_scatter<KT>), dim3((unsigned)nb), dim3(RS_BT), 0, s, ka, va, kb, vb, n,
+                         shift, hist, (uint32_t)nb);
+      KT* tk = ka;
+      ka = kb;
+      kb = tk;
+      uint32_t* tv = va;
+      va = vb;
+      vb = tv;
+    }
+  }
+  *kout = ka;
+  *vout = va;
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Ordered-execution passes.
+// ---------------------------------------------------------------------------
+__global__ void k_access_group_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
+                                     uint64_t n, uint32_t* flags) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    bool start = (p == 0) || srow[p] != srow[p - 1];
+    if (!start) start = op_is_write(ops[sop[p]]) || op_is_write(ops[sop[p - 1]]);
+    flags[p] = start ? (uint32_t)p : 0u;
+  }
+}
+__global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
+                              uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
+                              uint64_t nrows, const uint32_t* op_tx, const uint32_t* tx_rank,
+                              const uint64_t* ops, unsigned long long* conflicts) {
+  uint64_t local = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t op = sop[p];
+    need[op] = gstart[p] + pos_base;
+    const bool row_start = (p == 0) || srow[p] != srow[p - 1];
+    if (row_start) {
+      // an index miss probes to the index length; such ops never execute
+      if (srow[p] < nrows) counters[srow[p]] = (uint32_t)p + pos_base;
+    } else if (tx_rank && conflicts) {
+      const uint32_t pop = sop[p - 1];
+      if ((op_is_write(ops[op]) || op_is_write(ops[pop])) &&
+          tx_rank[op_tx[op]] == tx_rank[op_tx[pop]])
+        local++;
+    }
+  }
+  if (conflicts && local) atomicAdd(conflicts, (unsigned long long)local);
+}
+
+__global__ void k_gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
+                            uint64_t nrows, uint32_t* boundary, uint32_t* rowstart) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const bool start = (p == 0) || srow[p] != srow[p - 1];
+    rowstart[p] = start ? (uint32_t)p : 0u;
+    // index misses (IDX_MISS sorts last) never execute and raise no rank
+    boundary[p] = (!start && srow[p] < nrows &&
+                   (op_is_write(ops[sop[p]]) || op_is_write(ops[sop[p - 1]]))) ? 1u : 0u;
+  }
+}
+__global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                           const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank) {
+  uint32_t mx = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t local = S[p] - S[SS[p]];
+    atomicMax(&rank[op_tx[sop[p]]], local);
+    mx = local > mx ? local : mx;
+  }
+  atomicMax(max_rank, mx);
+}
+__global__ void k_gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                                uint64_t n, int rank_bits, uint64_t* keys) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    keys[i] = ((uint64_t)rows[i] << rank_bits) | rank[op_tx[i]];
+}
+__global__ void k_u64_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)(keys[i] >> shift);
+}
+__global__ void k_rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t* hist) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntx;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&hist[rank[i]], 1u);
+}
+
+__global__ void k_reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
+                             uint32_t* vprev) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rows[i];
+    if (r >= nrows_table) continue;
+    if (cc) dev::st(cc + r, 0);
+    if (vprev) dev::st32(vprev + r, 0);
+  }
+}
+__global__ void k_batch_begin(uint64_t* ctrs) {
+  if (threadIdx.x == 0) {
+    dev::st(ctrs + 0, 1);  // timestamp allocator restarts at 1 (timestamp.cuh:76-80)
+    dev::st(ctrs + 1, 0);  // commit tickets
+    dev::st(ctrs + 2, 0);  // MVCC pool bump pointer
+  }
+}
+
+}  // namespace
+
+namespace launch {
+
+#define GC_LAUNCH(kern, n, ...)                                                    \
+  do {                                                                            \
+    hipLaunchKernelGGL(kern, dim3(grid_stride_blocks(n)), dim3(BT), 0, s, __VA_ARGS__); \
+    return hipGetLastError();                                                     \
+  } while (0)
+
+hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s) {
+  GC_LAUNCH(k_seed_table, rows * cols, cells, rows, cols, seed);
+}
+hipError_t build_leaves(uint32_t* leaves, const uint32_t* keys, const uint32_t* vals, uint64_t n,
+                        uint64_t nleaves, hipStream_t s) {
+  GC_LAUNCH(k_build_leaves, nleaves * IDX_LEAF, leaves, keys, vals, n, nleaves);
+}
+hipError_t build_level_from_leaves(uint32_t* dst, const uint32_t* leaves, uint64_t nleaves,
+                                   uint64_t padded, hipStream_t s) {
+  GC_LAUNCH(k_level_from_leaves, padded, dst, leaves, nleaves, padded);
+}
+hipError_t build_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t nsrc_nodes,
+                                  uint64_t padded, hipStream_t s) {
+  GC_LAUNCH(k_level_from_level, padded, dst, src, nsrc_nodes, padded);
+}
+hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
+                 hipStream_t s) {
+  GC_LAUNCH(k_probe, n, ix, ops, n, rows_out);
+}
+hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s) {
+  GC_LAUNCH(k_op_tx, ntx, tx_off, ntx, op_tx);
+}
+hipError_t iota_u32(uint32_t* v, uint64_t n, uint32_t base, hipStream_t s) {
+  GC_LAUNCH(k_iota, n, v, n, base);
+}
+hipError_t fill_u32(uint32_t* v, uint64_t n, uint32_t x, hipStream_t s) { GC_LAUNCH(k_fill32, n, v, n, x); }
+hipError_t fill_u64(uint64_t* v, uint64_t n, uint64_t x, hipStream_t s) { GC_LAUNCH(k_fill64, n, v, n, x); }
+
+uint64_t scan_tmp_words(uint64_t n) { return (n + SC_TILE - 1) / SC_TILE + 64; }
+uint64_t radix_sort_tmp_words(uint64_t n) {
+  const uint64_t nb = (n + RS_TILE - 1) / RS_TILE;
+  return 256 * nb + scan_tmp_words(256 * nb) + 64;
+}
+hipError_t radix_sort_u32(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint64_t n,
+                          int key_bits, uint32_t* tmp, uint32_t** keys_out, uint32_t** vals_out,
+                          hipStream_t s) {
+  return radix_impl<uint32_t>(k0, v0, k1, v1, n, key_bits, tmp, keys_out, vals_out, s);
+}
+hipError_t radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t n,
+                          int key_bits, uint32_t* tmp, uint64_t** keys_out, uint32_t** vals_out,
+                          hipStream_t s) {
+  return radix_impl<uint64_t>(k0, v0, k1, v1, n, key_bits, tmp, keys_out, vals_out, s);
+}
+hipError_t exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
+  return scan_impl<OpAdd, false>(in, out, n, tmp, s);
+}
+hipError_t inclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
+  return scan_impl<OpAdd, true>(in, out, n, tmp, s);
+}
+hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
+  return scan_impl<OpMax, true>(in, out, n, tmp, s);
+}
+hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
+                              uint64_t n, uint32_t* flags, hipStream_t s) {
+  GC_LAUNCH(k_access_group_flags, n, srow, sop, ops, n, flags);
+}
+hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
+                       uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
+                       uint64_t nrows, const uint32_t* op_tx, const uint32_t* tx_rank,
+                       const uint64_t* ops, uint64_t* same_order_conflicts, hipStream_t s) {
+  GC_LAUNCH(k_access_need, n, srow, sop, gstart, n, pos_base, need, counters, nrows, op_tx, tx_rank,
+            ops, (unsigned long long*)same_order_conflicts);
+}
+hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
+                     uint64_t nrows, uint32_t* boundary, uint32_t* rowstart, hipStream_t s) {
+  GC_LAUNCH(k_gtx_flags, n, srow, sop, ops, n, nrows, boundary, rowstart);
+}
+hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                    const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s) {
+  GC_LAUNCH(k_gtx_rank, n, sop, S, SS, n, op_tx, rank, max_rank);
+}
+hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                         uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s) {
+  GC_LAUNCH(k_gtx_exec_keys, n, rows, op_tx, rank, n, rank_bits, keys);
+}
+hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s) {
+  GC_LAUNCH(k_u64_high, n, keys, n, shift, out);
+}
+hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t* hist, hipStream_t s) {
+  GC_LAUNCH(k_rank_hist, ntx, rank, ntx, hist);
+}
+hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
+                      uint32_t* vprev, hipStream_t s) {
+  GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, cc, vprev);
+}
+hipError_t batch_begin(uint64_t* ctrs, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, s, ctrs);
+  return hipGetLastError();
+}
+
+}  // namespace launch
+}  // namespace gcctb

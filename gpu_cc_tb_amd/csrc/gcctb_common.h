@@ -1,0 +1,213 @@
+// Shared host/device definitions of the YCSB transaction-execution path:
+// op encoding, deterministic hashes, CC word layouts and the kernel argument
+// block.  Compiled by hipcc for both the host library and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GC_HD __host__ __device__ __forceinline__
+#else
+#define GC_HD static inline
+#endif
+
+namespace gcctb {
+
+// ---------------------------------------------------------------------------
+// Scheme identifiers.  Names match the driver's cc_scheme argument
+// (benchmark_cpu/ycsb.cu:166-195).
+// ---------------------------------------------------------------------------
+enum Scheme : int32_t {
+  SCHEME_TPL_NW = 0,
+  SCHEME_TPL_WD = 1,
+  SCHEME_TO = 2,
+  SCHEME_MVCC = 3,
+  SCHEME_SILO = 4,
+  SCHEME_TICTOC = 5,
+  SCHEME_GACCO = 6,
+  SCHEME_GPUTX = 7,
+  SCHEME_COUNT = 8
+};
+
+// Longest transaction the kernels accept.  The reference's YCSBTx holds at most
+// 16 requests (include/ycsb.cuh:27); the variable-length stream goes to 32.
+constexpr int MAX_OPS = 32;
+// Columns per row (10 x u64 cells, 80 B).
+constexpr int DEFAULT_COLS = 10;
+
+// ---------------------------------------------------------------------------
+// Op encoding: one u64 per op.
+//   bits  0..31  key (looked up in the sorted-array index)
+//   bits 32..39  column
+//   bit  40      1 = write, 0 = read
+// The reference's YCSBReq is {int read; u32 key} (include/ycsb.cuh:21-25);
+// the column is the per-column extension of the u64-cell table.
+// ---------------------------------------------------------------------------
+GC_HD uint64_t make_op(uint32_t key, uint32_t col, bool is_write) {
+  return (uint64_t)key | ((uint64_t)(col & 0xffu) << 32) | ((uint64_t)(is_write ? 1 : 0) << 40);
+}
+GC_HD uint32_t op_key(uint64_t op) { return (uint32_t)op; }
+GC_HD uint32_t op_col(uint64_t op) { return (uint32_t)(op >> 32) & 0xffu; }
+GC_HD bool op_is_write(uint64_t op) { return (op >> 40) & 1u; }
+
+// ---------------------------------------------------------------------------
+// Deterministic hashing.  splitmix64 (Steele, Lea, Flood: "Fast splittable
+// pseudorandom number generators", OOPSLA 2014).
+// ---------------------------------------------------------------------------
+GC_HD uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+GC_HD uint64_t splitmix64(uint64_t x) { return mix64(x + 0x9E3779B97F4A7C15ull); }
+// Counter-based stream: (seed, a, b) -> u64.
+GC_HD uint64_t hash3(uint64_t seed, uint64_t a, uint64_t b) {
+  return splitmix64(splitmix64(seed ^ a) ^ b);
+}
+
+// Stream tags, xored into the seed so that the draws are independent.
+constexpr uint64_t TAG_KEY = 0x4b45590000000000ull;    // "KEY"
+constexpr uint64_t TAG_RW = 0x5257000000000000ull;     // "RW"
+constexpr uint64_t TAG_COL = 0x434f4c0000000000ull;    // "COL"
+constexpr uint64_t TAG_WRITE = 0x5752495445000000ull;  // "WRITE"
+constexpr uint64_t TAG_LEN = 0x4c454e0000000000ull;    // "LEN"
+constexpr uint64_t TAG_BACKOFF = 0x424f464600000000ull;
+
+// Initial cell value: splitmix64(seed ^ (row * cols + col)).
+GC_HD uint64_t cell_init(uint64_t seed, uint64_t row, uint32_t col, uint32_t cols) {
+  return splitmix64(seed ^ (row * (uint64_t)cols + col));
+}
+// Value written by op `op` (position inside the transaction) of transaction
+// `tx`.  It does not depend on the attempt nor on anything the transaction
+// read, so every committed attempt writes the same bits.
+GC_HD uint64_t write_value(uint64_t seed, uint64_t tx, uint32_t op) {
+  return hash3(seed ^ TAG_WRITE, tx, op);
+}
+
+// ---------------------------------------------------------------------------
+// CC word layouts (explicit masks; the reference uses LSB-first bitfields).
+// ---------------------------------------------------------------------------
+// 2PL lock word (include/twopl.cuh:10-21): bit 0 shared, bits 1..31
+// holder_cnt, bits 32..62 holder.
+namespace tpl {
+GC_HD uint64_t shared(uint64_t w) { return w & 1u; }
+GC_HD uint64_t cnt(uint64_t w) { return (w >> 1) & 0x7fffffffull; }
+GC_HD uint64_t holder(uint64_t w) { return (w >> 32) & 0x7fffffffull; }
+GC_HD uint64_t make(uint64_t sh, uint64_t c, uint64_t h) {
+  return (sh & 1u) | ((c & 0x7fffffffull) << 1) | ((h & 0x7fffffffull) << 32);
+}
+}  // namespace tpl
+
+// T/O and MVCC timestamp word (include/to.cuh:11-22, include/mvcc.cuh:12-23):
+// bit 0 uncommitted, bits 1..31 rts, bits 32..62 wts.
+namespace tsw {
+GC_HD uint64_t unc(uint64_t w) { return w & 1u; }
+GC_HD uint64_t rts(uint64_t w) { return (w >> 1) & 0x7fffffffull; }
+GC_HD uint64_t wts(uint64_t w) { return (w >> 32) & 0x7fffffffull; }
+GC_HD uint64_t make(uint64_t u, uint64_t r, uint64_t wt) {
+  return (u & 1u) | ((r & 0x7fffffffull) << 1) | ((wt & 0x7fffffffull) << 32);
+}
+constexpr uint64_t TS_MAX = 0x7fffffffull;  // 31-bit timestamps (to.cuh:108)
+}  // namespace tsw
+
+// Silo word (include/silo.cuh:17-32): bit 0 lock, bits 1..63 ts.
+namespace silo {
+GC_HD uint64_t locked(uint64_t w) { return w & 1u; }
+GC_HD uint64_t ts(uint64_t w) { return w >> 1; }
+GC_HD uint64_t make(uint64_t l, uint64_t t) { return (l & 1u) | (t << 1); }
+}  // namespace silo
+
+// TicToc word (include/tictoc.cuh:17-28): bit 0 lock, bits 1..15 delta,
+// bits 16..63 wts; rts = wts + delta.
+namespace tictoc {
+GC_HD uint64_t locked(uint64_t w) { return w & 1u; }
+GC_HD uint64_t delta(uint64_t w) { return (w >> 1) & 0x7fffull; }
+GC_HD uint64_t wts(uint64_t w) { return w >> 16; }
+GC_HD uint64_t rts(uint64_t w) { return wts(w) + delta(w); }
+GC_HD uint64_t make(uint64_t l, uint64_t d, uint64_t wt) {
+  return (l & 1u) | ((d & 0x7fffull) << 1) | (wt << 16);
+}
+}  // namespace tictoc
+
+// ---------------------------------------------------------------------------
+// Metrics: the reference's 7 counters (include/runtime.cuh:28-37) plus a few
+// build-specific ones.  Durations are in wall-clock ticks (s_memrealtime).
+// ---------------------------------------------------------------------------
+struct Metrics {
+  uint64_t abort;
+  uint64_t ts_duration;
+  uint64_t wait_duration;
+  uint64_t abort_duration;
+  uint64_t manager_duration;
+  uint64_t tot_duration;
+  uint64_t index_duration;
+  uint64_t committed;
+  uint64_t index_miss;
+  uint64_t error_flags;
+  uint64_t pad[6];
+};
+constexpr int METRICS_WORDS = 16;
+
+// error_flags bits
+constexpr uint64_t ERR_TS_OVERFLOW = 1;     // 31-bit timestamp space exhausted
+constexpr uint64_t ERR_POOL_OVERFLOW = 2;   // MVCC version pool full
+constexpr uint64_t ERR_CHAIN_BROKEN = 4;    // MVCC chain ended before a visible version
+constexpr uint64_t ERR_TXLEN = 8;           // transaction longer than the kernel's slots
+constexpr uint64_t ERR_WATCHDOG = 16;       // a wave exceeded the watchdog time and gave up
+
+// ---------------------------------------------------------------------------
+// Sorted-array index view (keys sorted ascending, leaves of 8 (key,val) pairs,
+// internal directory levels of 16 separators per 64-byte node).
+// ---------------------------------------------------------------------------
+constexpr uint32_t IDX_MISS = 0xffffffffu;  // device-side probe result for an absent key
+constexpr int IDX_LEAF = 8;
+constexpr int IDX_FANOUT = 16;
+constexpr int IDX_MAX_LEVELS = 12;
+struct IndexView {
+  const uint32_t* leaves;                 // nleaves * 16 u32: [8 keys][8 vals]
+  const uint32_t* level[IDX_MAX_LEVELS];  // level[0] = separators over leaves
+  uint32_t nlevels;                       // internal levels; level[nlevels-1] is one node
+  uint32_t length;                        // number of keys; Find miss returns length
+};
+
+// ---------------------------------------------------------------------------
+// Kernel argument block for the execution kernels.
+// ---------------------------------------------------------------------------
+struct ExecArgs {
+  // table (row-major u64 cells)
+  uint64_t* cells;
+  uint64_t rows;
+  uint32_t cols;
+  uint32_t scheme;
+  // index
+  IndexView index;
+  // transactions (CSR)
+  const uint64_t* ops;
+  const uint64_t* tx_off;
+  const uint32_t* tx_map;  // launch slot -> transaction id (null: identity)
+  uint64_t slot_lo;        // launch covers slots [slot_lo, slot_hi)
+  uint64_t slot_hi;
+  // CC state
+  uint64_t* cc;            // per-row CC word
+  uint32_t* vprev;         // MVCC: per-row head of the version chain (node+1)
+  uint64_t* pool;          // MVCC: version nodes, 3 u64 each
+  uint64_t pool_cap;       // MVCC: node capacity
+  uint32_t* counters;      // gacco/gputx: per-row completed-access counter
+  const uint32_t* need;    // gacco/gputx: per-op access-group start
+  uint64_t* ctrs;          // [0] ts allocator, [1] ticket, [2] pool top
+  // outputs
+  uint32_t* rows_out;      // per op: row the index returned
+  uint64_t* reads;         // per op: value returned by a read
+  uint32_t* obatch;        // per tx: batch id
+  uint64_t* okey0;         // per tx: serialization key (primary)
+  uint64_t* okey1;         // per tx: serialization key (secondary)
+  uint32_t* tx_aborts;     // per tx: aborted attempts
+  Metrics* metrics;
+  uint64_t seed;
+  uint32_t batch_id;
+  uint32_t backoff_ticks;  // randomized backoff scale after an abort (0 = none)
+  const uint32_t* tx_rank; // gputx: per-tx rank (okey0)
+  uint64_t watchdog_ticks; // a wave gives up after this many ticks (0 = never)
+};
+
+}  // namespace gcctb

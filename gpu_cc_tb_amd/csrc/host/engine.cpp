@@ -1,0 +1,813 @@
+// Host engine of the YCSB transaction-execution path: the context behind the C
+// ABI, the per-scheme host classes (the counterpart of the reference's
+// ConcurrencyControlCPUBase plugins, include/cc.cuh:18-56), and the plain
+// per-batch launch loop (benchmark_cpu/ycsb.cu:242-291).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../device/launch.h"
+#include "../gcctb_common.h"
+#include "host_internal.h"
+
+namespace gcctb {
+
+static thread_local std::string g_err;
+int set_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+namespace {
+
+struct HipFailure {
+  hipError_t err;
+  std::string where;
+};
+#define HIP_TRY(expr)                                                         \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) throw HipFailure{_e, std::string(#expr) + " @" + std::to_string(__LINE__)}; \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  uint64_t n = 0;
+  void alloc(uint64_t count) {
+    if (p && count <= n) return;
+    release();
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+int scheme_from_name(const char* s) {
+  static const char* names[SCHEME_COUNT] = {"tpl_nw", "tpl_wd", "to",     "mvcc",
+                                            "silo",   "tictoc", "gacco", "gputx"};
+  if (!s) return -1;
+  for (int i = 0; i < SCHEME_COUNT; ++i)
+    if (std::strcmp(s, names[i]) == 0) return i;
+  return -1;
+}
+
+int bits_for(uint64_t maxval) {  // bits needed to represent values <= maxval
+  int b = 0;
+  while (b < 64 && (maxval >> b) != 0) ++b;
+  return b;
+}
+
+}  // namespace
+
+class SchemeHost;
+
+}  // namespace gcctb
+
+struct gcctb_ctx {
+  gcctb::Scheme scheme;
+  gcctb_config cfg;
+  int G = 1;
+  int threads = 1024;
+  uint64_t watchdog_ticks = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  // table + index
+  gcctb::DevBuf<uint64_t> cells;
+  gcctb::DevBuf<uint32_t> leaves;
+  gcctb::DevBuf<uint32_t> levels[gcctb::IDX_MAX_LEVELS];
+  gcctb::IndexView ix{};
+  bool table_ready = false;
+  // transactions
+  uint64_t ntx = 0, nops = 0, nreads = 0;
+  std::vector<uint64_t> h_tx_off;
+  gcctb::DevBuf<uint64_t> ops, tx_off;
+  gcctb::DevBuf<uint32_t> op_tx;
+  // CC state
+  gcctb::DevBuf<uint64_t> cc;
+  gcctb::DevBuf<uint32_t> vprev;
+  gcctb::DevBuf<uint64_t> pool;
+  uint64_t pool_cap = 0;
+  gcctb::DevBuf<uint32_t> counters;
+  gcctb::DevBuf<uint32_t> need;
+  gcctb::DevBuf<uint64_t> ctrs;
+  // outputs
+  gcctb::DevBuf<uint32_t> rows_out;
+  gcctb::DevBuf<uint64_t> reads;
+  gcctb::DevBuf<uint32_t> obatch;
+  gcctb::DevBuf<uint64_t> okey0, okey1;
+  gcctb::DevBuf<uint32_t> tx_aborts;
+  gcctb::DevBuf<gcctb::Metrics> metrics;
+  // preprocessing scratch
+  gcctb::DevBuf<uint32_t> sk0, sv0, sk1, sv1, flags, gstart, sort_tmp, S, SS;
+  gcctb::DevBuf<uint64_t> sk64a, sk64b;
+  gcctb::DevBuf<uint32_t> rank, tx_map, max_rank;
+  gcctb::DevBuf<uint64_t> conflicts;
+  uint64_t nranks = 0;
+  std::vector<uint32_t> h_rank_counts;
+  std::unique_ptr<gcctb::SchemeHost> host;
+};
+
+namespace gcctb {
+
+// ---------------------------------------------------------------------------
+// Scheme host classes.  Init0 splits the transactions into batches
+// (include/cc.cuh:29-35); Init prepares one batch (reset / preprocessing,
+// timed as "pre"); Launch runs the batch's execution kernel (timed as "proc");
+// Post restores the per-row words the batch touched (the reference memsets the
+// whole CC table per batch instead, e.g. include/to.cuh:316-321).
+// ---------------------------------------------------------------------------
+class SchemeHost {
+ public:
+  explicit SchemeHost(gcctb_ctx& c) : ctx(c) {}
+  virtual ~SchemeHost() = default;
+  virtual void Init0() {
+    batches.clear();
+    const uint64_t bs = ctx.cfg.batch_size;
+    for (uint64_t st = 0; st < ctx.ntx; st += bs) batches.push_back(std::min(bs, ctx.ntx - st));
+  }
+  virtual void Init(int batch_id, uint64_t batch_st) {
+    (void)batch_id;
+    (void)batch_st;
+    HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream));
+  }
+  virtual void Launch(int batch_id, uint64_t slot_lo, uint64_t slot_hi) = 0;
+  virtual void Post(int batch_id, uint64_t batch_st, uint64_t n) {
+    (void)batch_id;
+    const uint64_t o0 = ctx.h_tx_off[batch_st], o1 = ctx.h_tx_off[batch_st + n];
+    HIP_TRY(launch::reset_rows(ctx.rows_out.p + o0, o1 - o0, ctx.cfg.rows, ctx.cc.p,
+                               ctx.scheme == SCHEME_MVCC ? ctx.vprev.p : nullptr, ctx.stream));
+  }
+  // Device allocations of the scheme (GetMemSize, include/cc.cuh:39).
+  virtual uint64_t GetMemSize() const { return ctx.cc.n * 8; }
+  std::vector<uint64_t> batches;
+
+ protected:
+  ExecArgs args(int batch_id, uint64_t slot_lo, uint64_t slot_hi) const {
+    ExecArgs a{};
+    a.cells = ctx.cells.p;
+    a.rows = ctx.cfg.rows;
+    a.cols = ctx.cfg.cols;
+    a.scheme = (uint32_t)ctx.scheme;
+    a.index = ctx.ix;
+    a.ops = ctx.ops.p;
+    a.tx_off = ctx.tx_off.p;
+    a.tx_map = nullptr;
+    a.slot_lo = slot_lo;
+    a.slot_hi = slot_hi;
+    a.cc = ctx.cc.p;
+    a.vprev = ctx.vprev.p;
+    a.pool = ctx.pool.p;
+    a.pool_cap = ctx.pool_cap;
+    a.counters = ctx.counters.p;
+    a.need = ctx.need.p;
+    a.ctrs = ctx.ctrs.p;
+    a.rows_out = ctx.rows_out.p;
+    a.reads = ctx.reads.p;
+    a.obatch = ctx.obatch.p;
+    a.okey0 = ctx.okey0.p;
+    a.okey1 = ctx.okey1.p;
+    a.tx_aborts = ctx.tx_aborts.p;
+    a.metrics = ctx.metrics.p;
+    a.seed = ctx.cfg.seed;
+    a.batch_id = (uint32_t)batch_id;
+    a.backoff_ticks = ctx.cfg.backoff_ticks;
+    a.tx_rank = nullptr;
+    a.watchdog_ticks = ctx.watchdog_ticks;
+    return a;
+  }
+  gcctb_ctx& ctx;
+};
+
+class TwoPLHost : public SchemeHost {
+ public:
+  TwoPLHost(gcctb_ctx& c, bool wd) : SchemeHost(c), wait_die(wd) {}
+  void Launch(int b, uint64_t lo, uint64_t hi) override {
+    HIP_TRY(launch::exec_tpl(wait_die, ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  }
+  // Locks are free again once every transaction of the batch committed.
+  void Post(int, uint64_t, uint64_t) override {}
+  bool wait_die;
+};
+class TOHost : public SchemeHost {
+ public:
+  using SchemeHost::SchemeHost;
+  void Launch(int b, uint64_t lo, uint64_t hi) override {
+    HIP_TRY(launch::exec_to(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  }
+};
+class MVCCHost : public SchemeHost {
+ public:
+  using SchemeHost::SchemeHost;
+  void Launch(int b, uint64_t lo, uint64_t hi) override {
+    HIP_TRY(launch::exec_mvcc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  }
+  uint64_t GetMemSize() const override { return ctx.cc.n * 8 + ctx.vprev.n * 4 + ctx.pool.n * 8; }
+};
+class SiloHost : public SchemeHost {
+ public:
+  using SchemeHost::SchemeHost;
+  void Launch(int b, uint64_t lo, uint64_t hi) override {
+    HIP_TRY(launch::exec_silo(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  }
+};
+class TictocHost : public SchemeHost {
+ public:
+  using SchemeHost::SchemeHost;
+  void Launch(int b, uint64_t lo, uint64_t hi) override {
+    HIP_TRY(launch::exec_tictoc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  }
+};
+
+// GaccO: per-batch lock-queue construction (Gacco_CPU::Init,
+// include/gacco.cuh:244-339) as probe -> stable radix sort by row -> access
+// groups -> need / counters, all on the device with no host round trip.
+class GaccoHost : public SchemeHost {
+ public:
+  using SchemeHost::SchemeHost;
+  void Init(int batch_id, uint64_t batch_st) override {
+    SchemeHost::Init(batch_id, batch_st);
+    const uint64_t n_tx = batches[batch_id];
+    const uint64_t o0 = ctx.h_tx_off[batch_st], o1 = ctx.h_tx_off[batch_st + n_tx];
+    const uint64_t n = o1 - o0;
+    if (n == 0) return;
+    hipStream_t s = ctx.stream;
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.sk0.p, s));
+    HIP_TRY(launch::iota_u32(ctx.sv0.p, n, (uint32_t)o0, s));
+    uint32_t *srow, *sop;
+    HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.ix.length),
+                                   ctx.sort_tmp.p, &srow, &sop, s));
+    HIP_TRY(launch::access_group_flags(srow, sop, ctx.ops.p, n, ctx.flags.p, s));
+    HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
+    HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows,
+                                nullptr, nullptr, ctx.ops.p, nullptr, s));
+  }
+  void Launch(int b, uint64_t lo, uint64_t hi) override {
+    HIP_TRY(launch::exec_ordered(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  }
+  void Post(int, uint64_t, uint64_t) override {}
+  uint64_t GetMemSize() const override { return ctx.counters.n * 4 + ctx.need.n * 4; }
+};
+
+// GPUTx: one-shot ranking of every transaction (GPUTx_CPU::Init0,
+// include/gputx.cuh:213-299; gputx_pre2/pre3, src/gputx.cu:5-38), then one
+// launch per rank level in (rank, transaction id) order.
+class GPUTxHost : public SchemeHost {
+ public:
+  using SchemeHost::SchemeHost;
+  void Init0() override {
+    batches.clear();
+    const uint64_t n = ctx.nops, ntx = ctx.ntx;
+    hipStream_t s = ctx.stream;
+    // per-row access lists in (row, tx) order
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s));
+    HIP_TRY(hipMemcpyAsync(ctx.sk0.p, ctx.rows_out.p, n * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch::iota_u32(ctx.sv0.p, n, 0, s));
+    uint32_t *srow, *sop;
+    const int row_bits = bits_for(ctx.ix.length);
+    HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, row_bits, ctx.sort_tmp.p,
+                                   &srow, &sop, s));
+    // one-pass local ranks and rank[tx] = max (src/gputx.cu:5-27)
+    HIP_TRY(launch::gtx_flags(srow, sop, ctx.ops.p, n, ctx.cfg.rows, ctx.flags.p, ctx.gstart.p, s));
+    HIP_TRY(launch::inclusive_sum_u32(ctx.flags.p, ctx.S.p, n, ctx.sort_tmp.p, s));
+    HIP_TRY(launch::inclusive_max_u32(ctx.gstart.p, ctx.SS.p, n, ctx.sort_tmp.p, s));
+    HIP_TRY(launch::fill_u32(ctx.rank.p, ntx, 0, s));
+    HIP_TRY(launch::fill_u32(ctx.max_rank.p, 1, 0, s));
+    HIP_TRY(launch::gtx_rank(sop, ctx.S.p, ctx.SS.p, n, ctx.op_tx.p, ctx.rank.p, ctx.max_rank.p, s));
+    uint32_t max_rank = 0;
+    HIP_TRY(hipMemcpyAsync(&max_rank, ctx.max_rank.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ctx.nranks = n ? (uint64_t)max_rank + 1 : 0;
+    // per-rank counts (gputx_pre3, src/gputx.cu:30-38) -> batches
+    ctx.h_rank_counts.assign(ctx.nranks, 0);
+    if (ctx.nranks) {
+      DevBuf<uint32_t> hist;
+      hist.alloc(ctx.nranks);
+      HIP_TRY(launch::fill_u32(hist.p, ctx.nranks, 0, s));
+      HIP_TRY(launch::rank_hist(ctx.rank.p, ntx, hist.p, s));
+      HIP_TRY(hipMemcpyAsync(ctx.h_rank_counts.data(), hist.p, ctx.nranks * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+    } else if (ntx) {
+      // no ops at all: every transaction has rank 0
+      ctx.nranks = 1;
+      ctx.h_rank_counts.assign(1, (uint32_t)ntx);
+    }
+    for (uint32_t c : ctx.h_rank_counts) batches.push_back(c);
+    // transactions in (rank, tx) order: stable sort of ranks (include/gputx.cuh:281)
+    const int rank_bits = std::max(1, bits_for(ctx.nranks ? ctx.nranks - 1 : 0));
+    HIP_TRY(hipMemcpyAsync(ctx.sk0.p, ctx.rank.p, ntx * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch::iota_u32(ctx.sv0.p, ntx, 0, s));
+    uint32_t *srank, *stx;
+    HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, ntx, rank_bits, ctx.sort_tmp.p,
+                                   &srank, &stx, s));
+    HIP_TRY(hipMemcpyAsync(ctx.tx_map.p, stx, ntx * 4, hipMemcpyDeviceToDevice, s));
+    // execution order of every row: (row, rank, tx)
+    if (n) {
+      HIP_TRY(launch::gtx_exec_keys(ctx.rows_out.p, ctx.op_tx.p, ctx.rank.p, n, rank_bits, ctx.sk64a.p, s));
+      HIP_TRY(launch::iota_u32(ctx.sv0.p, n, 0, s));
+      uint64_t* skey;
+      uint32_t* sop2;
+      HIP_TRY(launch::radix_sort_u64(ctx.sk64a.p, ctx.sv0.p, ctx.sk64b.p, ctx.sv1.p, n, row_bits + rank_bits,
+                                     ctx.sort_tmp.p, &skey, &sop2, s));
+      HIP_TRY(launch::u64_key_high(skey, n, rank_bits, ctx.sk0.p, s));
+      HIP_TRY(launch::access_group_flags(ctx.sk0.p, sop2, ctx.ops.p, n, ctx.flags.p, s));
+      HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
+      HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
+      HIP_TRY(launch::access_need(ctx.sk0.p, sop2, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p,
+                                  ctx.cfg.rows, ctx.op_tx.p, ctx.rank.p, ctx.ops.p, ctx.conflicts.p, s));
+    }
+  }
+  void Init(int, uint64_t) override {}
+  void Launch(int, uint64_t lo, uint64_t hi) override {
+    ExecArgs a = args(0, lo, hi);
+    a.tx_map = ctx.tx_map.p;
+    a.tx_rank = ctx.rank.p;
+    HIP_TRY(launch::exec_ordered(ctx.G, a, hi - lo, ctx.threads, ctx.stream));
+  }
+  void Post(int, uint64_t, uint64_t) override {}
+  uint64_t GetMemSize() const override { return ctx.counters.n * 4 + ctx.need.n * 4 + ctx.rank.n * 8; }
+};
+
+namespace {
+
+int hip_fail(const HipFailure& f) {
+  return set_error(GCCTB_EDEVICE, std::string("HIP error: ") + hipGetErrorString(f.err) + " in " + f.where);
+}
+
+void build_index_levels(gcctb_ctx& c, uint64_t n) {
+  const uint64_t nleaves = (n + IDX_LEAF - 1) / IDX_LEAF;
+  uint64_t entries = nleaves;
+  int l = 0;
+  while (true) {
+    if (l >= IDX_MAX_LEVELS) throw HipFailure{hipErrorInvalidValue, "index too deep"};
+    const uint64_t nodes = (entries + IDX_FANOUT - 1) / IDX_FANOUT;
+    const uint64_t padded = nodes * IDX_FANOUT;
+    c.levels[l].alloc(padded);
+    if (l == 0)
+      HIP_TRY(launch::build_level_from_leaves(c.levels[0].p, c.leaves.p, nleaves, padded, c.stream));
+    else
+      HIP_TRY(launch::build_level_from_level(c.levels[l].p, c.levels[l - 1].p, entries, padded, c.stream));
+    c.ix.level[l] = c.levels[l].p;
+    ++l;
+    if (nodes == 1) break;
+    entries = nodes;
+  }
+  for (int k = l; k < IDX_MAX_LEVELS; ++k) {
+    c.levels[k].release();
+    c.ix.level[k] = nullptr;
+  }
+  c.ix.leaves = c.leaves.p;
+  c.ix.nlevels = (uint32_t)l;
+  c.ix.length = (uint32_t)n;
+}
+
+void build_index(gcctb_ctx& c, const uint32_t* dkeys, const uint32_t* dvals, uint64_t n) {
+  const uint64_t nleaves = (n + IDX_LEAF - 1) / IDX_LEAF;
+  c.leaves.alloc(nleaves * 2 * IDX_LEAF);
+  HIP_TRY(launch::build_leaves(c.leaves.p, dkeys, dvals, n, nleaves, c.stream));
+  build_index_levels(c, n);
+  HIP_TRY(hipStreamSynchronize(c.stream));
+}
+
+void ensure_state(gcctb_ctx& c) {
+  const uint64_t rows = c.cfg.rows;
+  c.cc.alloc(rows);
+  HIP_TRY(hipMemsetAsync(c.cc.p, 0, rows * 8, c.stream));
+  if (c.scheme == SCHEME_MVCC) {
+    c.vprev.alloc(rows);
+    HIP_TRY(hipMemsetAsync(c.vprev.p, 0, rows * 4, c.stream));
+  }
+  if (c.scheme == SCHEME_GACCO || c.scheme == SCHEME_GPUTX) {
+    c.counters.alloc(rows);
+    HIP_TRY(hipMemsetAsync(c.counters.p, 0, rows * 4, c.stream));
+  }
+  c.ctrs.alloc(8);
+  HIP_TRY(hipMemsetAsync(c.ctrs.p, 0, 64, c.stream));
+  c.metrics.alloc(1);
+}
+
+}  // namespace
+}  // namespace gcctb
+
+using namespace gcctb;
+
+extern "C" {
+
+const char* gcctb_last_error(void) { return g_err.c_str(); }
+
+int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
+  if (!cfg || !out) return set_error(GCCTB_EINVAL, "null argument");
+  *out = nullptr;
+  const int sch = scheme_from_name(cfg->cc_scheme);
+  if (sch < 0) return set_error(GCCTB_EINVAL, std::string("unknown cc_scheme ") + (cfg->cc_scheme ? cfg->cc_scheme : "(null)"));
+  // benchmark_cpu/ycsb.cu:97-98: warp_cnt > 32 || valid_txn_bitoffset > 5 -> -1
+  if (cfg->block_size < 1 || cfg->block_size > 32 || cfg->warp_density < 0 || cfg->warp_density > 5)
+    return set_error(GCCTB_EINVAL, "warp_density must be 0..5 and block_size 1..32");
+  if (cfg->batch_size < 1) return set_error(GCCTB_EINVAL, "batch_size must be >= 1");
+  if (cfg->rows < 1 || cfg->rows > 0xfffffffeull) return set_error(GCCTB_EINVAL, "rows out of range");
+  if (cfg->cols < 1 || cfg->cols > 255) return set_error(GCCTB_EINVAL, "cols out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(GCCTB_EDEVICE, "no HIP device: the transaction kernels need an MI355X (gfx950)");
+  auto* c = new (std::nothrow) gcctb_ctx();
+  if (!c) return set_error(GCCTB_ENOMEM, "out of host memory");
+  c->scheme = (Scheme)sch;
+  c->cfg = *cfg;
+  c->cfg.cc_scheme = nullptr;
+  c->G = 1 << cfg->warp_density;
+  c->threads = cfg->block_size * 32;
+  {
+    // watchdog: 20 s of the 100 MHz wall clock unless GCCTB_WATCHDOG_S says otherwise
+    const char* wd = std::getenv("GCCTB_WATCHDOG_S");
+    const double sec = wd ? std::atof(wd) : 20.0;
+    c->watchdog_ticks = sec > 0 ? (uint64_t)(sec * 1e8) : 0;
+  }
+  try {
+    HIP_TRY(hipSetDevice(0));
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+    switch (c->scheme) {
+      case SCHEME_TPL_NW: c->host.reset(new TwoPLHost(*c, false)); break;
+      case SCHEME_TPL_WD: c->host.reset(new TwoPLHost(*c, true)); break;
+      case SCHEME_TO: c->host.reset(new TOHost(*c)); break;
+      case SCHEME_MVCC: c->host.reset(new MVCCHost(*c)); break;
+      case SCHEME_SILO: c->host.reset(new SiloHost(*c)); break;
+      case SCHEME_TICTOC: c->host.reset(new TictocHost(*c)); break;
+      case SCHEME_GACCO: c->host.reset(new GaccoHost(*c)); break;
+      case SCHEME_GPUTX: c->host.reset(new GPUTxHost(*c)); break;
+      default: break;
+    }
+    c->cells.alloc(cfg->rows * cfg->cols);
+    ensure_state(*c);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } catch (const HipFailure& f) {
+    const int rc = hip_fail(f);
+    gcctb_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return GCCTB_OK;
+}
+
+void gcctb_destroy(gcctb_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->host.reset();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  hipStream_t s = c->stream;
+  delete c;  // DevBuf destructors free device memory
+  if (s) (void)hipStreamDestroy(s);
+}
+
+int gcctb_build_index_identity(gcctb_ctx* c) {
+  if (!c) return set_error(GCCTB_EINVAL, "null context");
+  try {
+    build_index(*c, nullptr, nullptr, c->cfg.rows);
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_build_index(gcctb_ctx* c, const uint32_t* keys, const uint32_t* vals, uint64_t n) {
+  if (!c || (!keys && n) || (!vals && n)) return set_error(GCCTB_EINVAL, "null argument");
+  if (n == 0 || n > 0xfffffffeull) return set_error(GCCTB_EINVAL, "index length out of range");
+  std::vector<std::pair<uint32_t, uint32_t>> kv(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    if (keys[i] == 0xffffffffu) return set_error(GCCTB_EINVAL, "key 0xffffffff is reserved");
+    kv[i] = {keys[i], vals[i]};
+  }
+  std::sort(kv.begin(), kv.end());  // SortedArray_CPU ctor sorts (include/index.cuh:106-108)
+  for (uint64_t i = 1; i < n; ++i)
+    if (kv[i].first == kv[i - 1].first) return set_error(GCCTB_EINVAL, "duplicate index key");
+  std::vector<uint32_t> k(n), v(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    k[i] = kv[i].first;
+    v[i] = kv[i].second;
+  }
+  try {
+    DevBuf<uint32_t> dk, dv;
+    dk.alloc(n);
+    dv.alloc(n);
+    HIP_TRY(hipMemcpyAsync(dk.p, k.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dv.p, v.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    build_index(*c, dk.p, dv.p, n);
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_probe(gcctb_ctx* c, const uint32_t* keys, uint64_t n, uint32_t* rows_out) {
+  if (!c || (n && (!keys || !rows_out))) return set_error(GCCTB_EINVAL, "null argument");
+  if (!c->ix.leaves) return set_error(GCCTB_ESTATE, "no index");
+  if (n == 0) return GCCTB_OK;
+  try {
+    std::vector<uint64_t> ops(n);
+    for (uint64_t i = 0; i < n; ++i) ops[i] = make_op(keys[i], 0, false);
+    DevBuf<uint64_t> dops;
+    DevBuf<uint32_t> drows;
+    dops.alloc(n);
+    drows.alloc(n);
+    HIP_TRY(hipMemcpyAsync(dops.p, ops.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch::probe(c->ix, dops.p, n, drows.p, c->stream));
+    HIP_TRY(hipMemcpyAsync(rows_out, drows.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // Find's contract: a miss returns the index length (include/index.cuh:89-90)
+    for (uint64_t i = 0; i < n; ++i)
+      if (rows_out[i] == IDX_MISS) rows_out[i] = c->ix.length;
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_load_table(gcctb_ctx* c, const uint64_t* cells, uint64_t rows, uint32_t cols) {
+  if (!c || !cells) return set_error(GCCTB_EINVAL, "null argument");
+  if (rows != c->cfg.rows || cols != c->cfg.cols) return set_error(GCCTB_EINVAL, "table shape differs from the config");
+  try {
+    HIP_TRY(hipMemcpyAsync(c->cells.p, cells, rows * cols * 8, hipMemcpyHostToDevice, c->stream));
+    ensure_state(*c);
+    if (!c->ix.leaves) build_index(*c, nullptr, nullptr, rows);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  c->table_ready = true;
+  return GCCTB_OK;
+}
+
+int gcctb_seed_table(gcctb_ctx* c) {
+  if (!c) return set_error(GCCTB_EINVAL, "null context");
+  try {
+    HIP_TRY(launch::seed_table(c->cells.p, c->cfg.rows, c->cfg.cols, c->cfg.seed, c->stream));
+    ensure_state(*c);
+    if (!c->ix.leaves) build_index(*c, nullptr, nullptr, c->cfg.rows);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  c->table_ready = true;
+  return GCCTB_OK;
+}
+
+int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx) {
+  if (!c || !tx_off || (!ops && ntx)) return set_error(GCCTB_EINVAL, "null argument");
+  if (ntx >= (1ull << 31)) return set_error(GCCTB_EINVAL, "at most 2^31-1 transactions (31-bit lock holder ids)");
+  if (tx_off[0] != 0) return set_error(GCCTB_EINVAL, "tx_off[0] must be 0");
+  uint64_t nreads = 0;
+  for (uint64_t t = 0; t < ntx; ++t) {
+    if (tx_off[t + 1] < tx_off[t]) return set_error(GCCTB_EINVAL, "tx_off must be non-decreasing");
+    const uint64_t len = tx_off[t + 1] - tx_off[t];
+    if (len > (uint64_t)MAX_OPS) return set_error(GCCTB_EINVAL, "a transaction has more than 32 ops");
+    for (uint64_t i = tx_off[t]; i < tx_off[t + 1]; ++i) {
+      if (op_col(ops[i]) >= c->cfg.cols) return set_error(GCCTB_EINVAL, "op column out of range");
+      nreads += op_is_write(ops[i]) ? 0 : 1;
+    }
+  }
+  const uint64_t nops = tx_off[ntx];
+  if (nops >= (1ull << 32)) return set_error(GCCTB_EINVAL, "at most 2^32-1 ops");
+  // MVCC version pool: the largest number of writes in one batch
+  uint64_t cap = 0;
+  if (c->scheme == SCHEME_MVCC) {
+    for (uint64_t st = 0; st < ntx; st += c->cfg.batch_size) {
+      const uint64_t en = std::min(ntx, st + c->cfg.batch_size);
+      uint64_t w = 0;
+      for (uint64_t i = tx_off[st]; i < tx_off[en]; ++i) w += op_is_write(ops[i]) ? 1 : 0;
+      cap = std::max(cap, w);
+    }
+  }
+  try {
+    c->ntx = ntx;
+    c->nops = nops;
+    c->nreads = nreads;
+    c->h_tx_off.assign(tx_off, tx_off + ntx + 1);
+    c->ops.alloc(nops);
+    c->tx_off.alloc(ntx + 1);
+    c->op_tx.alloc(nops);
+    c->rows_out.alloc(nops);
+    c->reads.alloc(nops);
+    c->obatch.alloc(ntx);
+    c->okey0.alloc(ntx);
+    c->okey1.alloc(ntx);
+    c->tx_aborts.alloc(ntx);
+    if (nops) HIP_TRY(hipMemcpyAsync(c->ops.p, ops, nops * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tx_off.p, tx_off, (ntx + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(c->reads.p, 0, nops * 8 + 8, c->stream));
+    HIP_TRY(hipMemsetAsync(c->tx_aborts.p, 0, ntx * 4 + 4, c->stream));
+    HIP_TRY(launch::op_tx(c->tx_off.p, ntx, c->op_tx.p, c->stream));
+    if (c->scheme == SCHEME_MVCC) {
+      c->pool_cap = cap;
+      c->pool.alloc(3 * cap + 3);
+    }
+    if (c->scheme == SCHEME_GACCO || c->scheme == SCHEME_GPUTX) {
+      const uint64_t n = c->scheme == SCHEME_GACCO
+                             ? [&] {
+                                 uint64_t m = 0;
+                                 for (uint64_t st = 0; st < ntx; st += c->cfg.batch_size) {
+                                   const uint64_t en = std::min(ntx, st + c->cfg.batch_size);
+                                   m = std::max(m, tx_off[en] - tx_off[st]);
+                                 }
+                                 return m;
+                               }()
+                             : std::max(nops, ntx);
+      c->need.alloc(nops);
+      c->sk0.alloc(n);
+      c->sv0.alloc(n);
+      c->sk1.alloc(n);
+      c->sv1.alloc(n);
+      c->flags.alloc(n);
+      c->gstart.alloc(n);
+      c->sort_tmp.alloc(std::max(launch::radix_sort_tmp_words(n), launch::scan_tmp_words(n)));
+      if (c->scheme == SCHEME_GPUTX) {
+        c->S.alloc(n);
+        c->SS.alloc(n);
+        c->sk64a.alloc(n);
+        c->sk64b.alloc(n);
+        c->rank.alloc(ntx);
+        c->tx_map.alloc(ntx);
+        c->max_rank.alloc(1);
+        c->conflicts.alloc(1);
+      }
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
+  if (!c) return set_error(GCCTB_EINVAL, "null context");
+  if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
+  gcctb_stats st{};
+  Metrics m{};
+  try {
+    hipStream_t s = c->stream;
+    HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics), s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0;
+    double pre = 0, proc = 0;
+    HIP_TRY(hipEventRecord(c->ev[0], s));
+    c->host->Init0();
+    HIP_TRY(hipEventRecord(c->ev[1], s));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    pre += ms;
+    uint64_t batch_st = 0;
+    uint64_t launches = 0;
+    const auto& batches = c->host->batches;
+    for (size_t i = 0; i < batches.size(); ++i) {
+      const uint64_t n = batches[i];
+      if (!n) continue;  // benchmark_cpu/ycsb.cu:266-267
+      HIP_TRY(hipEventRecord(c->ev[2], s));
+      c->host->Init((int)i, batch_st);
+      HIP_TRY(hipEventRecord(c->ev[3], s));
+      c->host->Launch((int)i, batch_st, batch_st + n);
+      HIP_TRY(hipEventRecord(c->ev[4], s));
+      c->host->Post((int)i, batch_st, n);
+      HIP_TRY(hipEventRecord(c->ev[5], s));
+      HIP_TRY(hipEventSynchronize(c->ev[5]));
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+      pre += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+      pre += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+      proc += ms;
+      batch_st += n;
+      launches++;
+    }
+    HIP_TRY(hipEventRecord(c->ev[6], s));
+    HIP_TRY(hipEventSynchronize(c->ev[6]));
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[6]));
+    st.tot_ms = ms;
+    st.pre_ms = pre;
+    st.proc_ms = proc;
+    st.exec_kernel_ms = proc;
+    st.batches = launches;
+    HIP_TRY(hipMemcpy(&m, c->metrics.p, sizeof(Metrics), hipMemcpyDeviceToHost));
+    if (c->scheme == SCHEME_GPUTX) {
+      st.ranks = c->nranks;
+      if (c->nops) HIP_TRY(hipMemcpy(&st.same_rank_conflicts, c->conflicts.p, 8, hipMemcpyDeviceToHost));
+    }
+    int rate = 0;
+    HIP_TRY(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0));
+    st.tick_khz = rate;
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  st.txs = c->ntx;
+  st.committed = m.committed;
+  st.aborts = m.abort;
+  st.ts_ticks = m.ts_duration;
+  st.wait_ticks = m.wait_duration;
+  st.abort_ticks = m.abort_duration;
+  st.manager_ticks = m.manager_duration;
+  st.tot_ticks = m.tot_duration;
+  st.index_ticks = m.index_duration;
+  st.index_miss = m.index_miss;
+  st.error_flags = m.error_flags;
+  st.total_ops = c->nops;
+  st.read_ops = c->nreads;
+  if (out) *out = st;
+  if (m.error_flags) return set_error(GCCTB_EKERNEL, "kernel error flags 0x" + std::to_string(m.error_flags));
+  if (m.committed != c->ntx) return set_error(GCCTB_EKERNEL, "not every transaction committed");
+  return GCCTB_OK;
+}
+
+int gcctb_fetch(gcctb_ctx* c, uint64_t* table, uint64_t* reads, uint64_t* order, uint32_t* aborts) {
+  if (!c) return set_error(GCCTB_EINVAL, "null context");
+  try {
+    hipStream_t s = c->stream;
+    HIP_TRY(hipStreamSynchronize(s));
+    if (table) HIP_TRY(hipMemcpy(table, c->cells.p, c->cfg.rows * c->cfg.cols * 8, hipMemcpyDeviceToHost));
+    if (reads && c->nops) HIP_TRY(hipMemcpy(reads, c->reads.p, c->nops * 8, hipMemcpyDeviceToHost));
+    if (aborts && c->ntx) HIP_TRY(hipMemcpy(aborts, c->tx_aborts.p, c->ntx * 4, hipMemcpyDeviceToHost));
+    if (order && c->ntx) {
+      std::vector<uint32_t> b(c->ntx);
+      std::vector<uint64_t> k0(c->ntx), k1(c->ntx);
+      HIP_TRY(hipMemcpy(b.data(), c->obatch.p, c->ntx * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(k0.data(), c->okey0.p, c->ntx * 8, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(k1.data(), c->okey1.p, c->ntx * 8, hipMemcpyDeviceToHost));
+      for (uint64_t t = 0; t < c->ntx; ++t) {
+        order[3 * t + 0] = b[t];
+        order[3 * t + 1] = k0[t];
+        order[3 * t + 2] = k1[t];
+      }
+    }
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_fetch_ranks(gcctb_ctx* c, uint32_t* rank, uint32_t* counts, uint64_t* nranks) {
+  if (!c) return set_error(GCCTB_EINVAL, "null context");
+  if (c->scheme != SCHEME_GPUTX) return set_error(GCCTB_ESTATE, "ranks exist only for gputx");
+  try {
+    if (rank && c->ntx) HIP_TRY(hipMemcpy(rank, c->rank.p, c->ntx * 4, hipMemcpyDeviceToHost));
+    if (counts) std::memcpy(counts, c->h_rank_counts.data(), c->h_rank_counts.size() * 4);
+    if (nranks) *nranks = c->nranks;
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_fetch_cc(gcctb_ctx* c, uint64_t* words) {
+  if (!c || !words) return set_error(GCCTB_EINVAL, "null argument");
+  try {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(words, c->cc.p, c->cfg.rows * 8, hipMemcpyDeviceToHost));
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_generate(const gcctb_workload* w, uint64_t* ops, uint64_t* tx_off, uint64_t* nops) {
+  if (!w || !tx_off || (!ops && w->ntx)) return set_error(GCCTB_EINVAL, "null argument");
+  return generate(*w, ops, tx_off, nops);
+}
+
+double gcctb_zeta(uint64_t n, double theta) { return zeta(n, theta, 0); }
+
+int gcctb_read_txs_file(const char* path, uint64_t ntx, uint32_t cols, uint64_t seed, uint64_t* ops,
+                        uint64_t* tx_off, uint64_t* nops) {
+  if (!path || !ops || !tx_off) return set_error(GCCTB_EINVAL, "null argument");
+  return read_txs_file(path, ntx, cols, seed, ops, tx_off, nops);
+}
+
+int gcctb_write_txs_file(const char* path, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx) {
+  if (!path || !tx_off || (!ops && ntx)) return set_error(GCCTB_EINVAL, "null argument");
+  return write_txs_file(path, ops, tx_off, ntx);
+}
+
+int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int32_t* cus) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(GCCTB_EDEVICE, "no HIP device");
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, 0) != hipSuccess) return set_error(GCCTB_EDEVICE, "hipGetDeviceProperties failed");
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  int rate = 0;
+  (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0);
+  if (wall_clock_khz) *wall_clock_khz = rate;
+  if (cus) *cus = p.multiProcessorCount;
+  return GCCTB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,17 @@
+// Internal declarations of the host library.
+#pragma once
+#include <string>
+#include "../../../include/gcctb.h"
+
+namespace gcctb {
+
+// Records the message for gcctb_last_error() and returns `code`.
+int set_error(int code, const std::string& msg);
+
+double zeta(uint64_t n, double theta, int threads);
+int generate(const gcctb_workload& w, uint64_t* ops, uint64_t* tx_off, uint64_t* nops);
+int read_txs_file(const char* path, uint64_t ntx, uint32_t cols, uint64_t seed, uint64_t* ops,
+                  uint64_t* tx_off, uint64_t* nops);
+int write_txs_file(const char* path, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx);
+
+}  // namespace gcctb

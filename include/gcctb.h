@@ -1,0 +1,157 @@
+/*
+ * gcctb.h -- C ABI of the MI355X (gfx950) YCSB transaction-execution path.
+ *
+ * Plain pointers and sizes only; every call returns GCCTB_OK (0) or a negative
+ * error code, with the message available from gcctb_last_error().  A context
+ * owns every device allocation it makes and is used by one host thread at a
+ * time.  Contexts on the same GPU are independent.
+ *
+ * Each entry point names the reference interface it replaces (paths are in
+ * Al0ha0e/gpu_cc_tb).
+ */
+#ifndef GCCTB_H
+#define GCCTB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCCTB_OK 0
+#define GCCTB_EINVAL (-1)   /* bad argument (the reference's driver returns -1, benchmark_cpu/ycsb.cu:97-98) */
+#define GCCTB_ENOMEM (-2)   /* device or host allocation failed */
+#define GCCTB_EDEVICE (-3)  /* HIP runtime error, missing GPU or missing kernel image */
+#define GCCTB_ESTATE (-4)   /* call out of order (e.g. run before load_ops) */
+#define GCCTB_EIO (-5)      /* file could not be read */
+#define GCCTB_EKERNEL (-6)  /* a kernel reported an error flag (timestamp/pool overflow) */
+
+typedef struct gcctb_ctx gcctb_ctx;
+
+/* Scheme and launch parameters.  cc_scheme, warp_density, block_size and
+ * batch_size keep the meaning of the reference driver's argv[4..7]
+ * (benchmark_cpu/ycsb.cu:90-98): a transaction owns 2^warp_density lanes, a
+ * workgroup has block_size*32 threads, and transactions run in batches of
+ * batch_size. */
+typedef struct gcctb_config {
+  const char* cc_scheme;   /* tpl_nw tpl_wd to mvcc silo tictoc gacco gputx */
+  int32_t warp_density;    /* 0..5 */
+  int32_t block_size;      /* 1..32 */
+  uint64_t batch_size;     /* >= 1 */
+  uint64_t rows;           /* table rows (<= 2^32 - 2) */
+  uint32_t cols;           /* u64 columns per row (1..255) */
+  uint64_t seed;           /* seeds the table and the written values */
+  uint32_t backoff_ticks;  /* randomized retry backoff scale (10 ns ticks), 0 = immediate retry */
+  uint32_t reserved;
+} gcctb_config;
+
+/* Result of one run.  The first block mirrors the reference's Metrics and
+ * ExecInfo timings (include/runtime.cuh:28-37,114-131). */
+typedef struct gcctb_stats {
+  uint64_t txs;            /* transactions executed */
+  uint64_t committed;      /* transactions committed (== txs) */
+  uint64_t aborts;         /* aborted attempts */
+  uint64_t ts_ticks, wait_ticks, abort_ticks, manager_ticks, tot_ticks, index_ticks;
+  double tick_khz;         /* frequency of the tick counters */
+  double pre_ms;           /* per-batch reset / preprocessing (reference "pre") */
+  double proc_ms;          /* execution kernels (reference "proc") */
+  double tot_ms;           /* whole timed region (reference "tot") */
+  double exec_kernel_ms;   /* sum of execution-kernel durations (HIP events) */
+  uint64_t batches;        /* launches of the execution kernel */
+  uint64_t ranks;          /* gputx: number of rank levels */
+  uint64_t same_rank_conflicts; /* gputx: adjacent conflicting same-rank accesses */
+  uint64_t index_miss;     /* ops whose key was absent from the index */
+  uint64_t error_flags;    /* kernel error bits (0 = none) */
+  uint64_t total_ops;      /* ops executed */
+  uint64_t read_ops;       /* read ops executed */
+} gcctb_stats;
+
+/* Create a context: replaces constructing the scheme by name and its device
+ * allocations (benchmark_cpu/ycsb.cu:166-195; ConcurrencyControlCPUBase ctor,
+ * include/cc.cuh:23-27; TwoPL_CPU ctor, include/twopl.cuh:400-429, etc.). */
+int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out);
+void gcctb_destroy(gcctb_ctx* ctx);
+const char* gcctb_last_error(void);
+
+/* Upload a row-major rows x cols table of u64 cells: replaces
+ * Table_CPU::InitData + DB_CPU::ToGPU (include/table.cuh:67-75,263-276,
+ * include/db.cuh:97-126).  rows/cols must match the config. */
+int gcctb_load_table(gcctb_ctx* ctx, const uint64_t* cells, uint64_t rows, uint32_t cols);
+/* Seed the table on the device: cell[r][c] = splitmix64(seed ^ (r*cols + c)). */
+int gcctb_seed_table(gcctb_ctx* ctx);
+
+/* Build the sorted-array index over the id column (key i -> row i):
+ * replaces Table_CPU::AddIndex(SORTED_ARRAY, 0) (include/table.cuh:95-130,
+ * include/index.cuh:96-111).  Called by load_table/seed_table. */
+int gcctb_build_index_identity(gcctb_ctx* ctx);
+/* Build the index from host (key, value) pairs (any order; keys distinct,
+ * keys < 2^32-1, values < rows). */
+int gcctb_build_index(gcctb_ctx* ctx, const uint32_t* keys, const uint32_t* vals, uint64_t n);
+/* Index probe for tests: rows_out[i] = Find(keys[i]) (include/index.cuh:74-91);
+ * a miss returns the index length. */
+int gcctb_probe(gcctb_ctx* ctx, const uint32_t* keys, uint64_t n, uint32_t* rows_out);
+
+/* Upload transactions in CSR form: ops[tx_off[t] .. tx_off[t+1]) are the ops
+ * of transaction t (encoding: key bits 0..31, column bits 32..39, write bit
+ * 40).  Replaces the .txs load, flattening and DynamicTransactionSet_CPU
+ * (benchmark_cpu/ycsb.cu:116-157, include/transaction.cuh:199-368). */
+int gcctb_load_ops(gcctb_ctx* ctx, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx);
+
+/* Run every loaded transaction with the reference's launch structure: Init0
+ * splits into batches, then per batch Init (reset / preprocess) + one kernel
+ * launch + synchronization (benchmark_cpu/ycsb.cu:249-291, include/cc.cuh:29-36,
+ * benchmark/ycsb.cu:21-111).  The table keeps the result; run again to
+ * continue on it. */
+int gcctb_run_plain(gcctb_ctx* ctx, gcctb_stats* out);
+
+/* Copy results back for checking.  Any pointer may be null.
+ *   table:  rows*cols cells
+ *   reads:  one u64 per op (value returned by a read; writes leave 0)
+ *   order:  3 u64 per transaction {batch, key0, key1}: the transaction's place
+ *           in the serialization order is ascending (batch, key0, key1)
+ *   aborts: one u32 per transaction */
+int gcctb_fetch(gcctb_ctx* ctx, uint64_t* table, uint64_t* reads, uint64_t* order, uint32_t* aborts);
+/* gputx: per-transaction rank (src/gputx.cu:5-27) and per-rank counts
+ * (include/gputx.cuh:283-294).  counts may be null; nranks receives the level
+ * count. */
+int gcctb_fetch_ranks(gcctb_ctx* ctx, uint32_t* rank, uint32_t* counts, uint64_t* nranks);
+/* Per-row CC words after a run (2PL/T-O/MVCC/Silo/TicToc: one u64 per row). */
+int gcctb_fetch_cc(gcctb_ctx* ctx, uint64_t* words);
+
+/* Workload generation (host, deterministic in (seed, tx, draw); counter-based,
+ * multi-threaded).  Zipf draw follows generation/gen_ycsb.cu:76-102 with
+ * n = rows and row = rank - 1; keys are distinct inside a transaction and ops
+ * are sorted by key (gen_ycsb.cu:129-150).  Each op reads with probability
+ * read_ratio, else writes; the column is drawn uniformly.
+ * ops must hold ntx*len_max entries, tx_off ntx+1.  Returns the op count in
+ * *nops. */
+typedef struct gcctb_workload {
+  uint64_t rows;
+  uint32_t cols;
+  uint64_t ntx;
+  uint64_t tx_base;        /* global id of the first transaction (stream position) */
+  uint32_t len_min;        /* ops per transaction: uniform in [len_min, len_max] */
+  uint32_t len_max;
+  double theta;            /* Zipf skew in [0, 1) */
+  double read_ratio;       /* probability that an op reads */
+  uint64_t seed;
+  int32_t threads;         /* host threads (0 = all cores) */
+} gcctb_workload;
+int gcctb_generate(const gcctb_workload* w, uint64_t* ops, uint64_t* tx_off, uint64_t* nops);
+/* zeta(n, theta) as the generator computes it (for tests). */
+double gcctb_zeta(uint64_t n, double theta);
+
+/* Read a reference .txs file (raw YCSBTx records of 136 bytes, include/ycsb.cuh:21-33)
+ * and convert it to CSR ops; the column of op j of transaction t is
+ * hash(seed, t, j) mod cols.  ops must hold 16*ntx entries. */
+int gcctb_read_txs_file(const char* path, uint64_t ntx, uint32_t cols, uint64_t seed, uint64_t* ops,
+                        uint64_t* tx_off, uint64_t* nops);
+int gcctb_write_txs_file(const char* path, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx);
+
+/* Device information. */
+int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int32_t* cus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GCCTB_H */

@@ -1,0 +1,84 @@
+/*
+ * oracle.h -- CPU oracle for the YCSB transaction-execution path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load liboracle.so, and only to check or time
+ * against the GPU path; the product library never links or calls it.
+ *
+ * Plain C11, built with gcc from this directory (see Makefile).  Every
+ * function names the reference code it restates.
+ */
+#ifndef GCCTB_ORACLE_H
+#define GCCTB_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- hashing (splitmix64, Steele et al. OOPSLA 2014) and seeded values ---- */
+uint64_t orc_splitmix64(uint64_t x);
+uint64_t orc_hash3(uint64_t seed, uint64_t a, uint64_t b);
+uint64_t orc_cell_init(uint64_t seed, uint64_t row, uint32_t col, uint32_t cols);
+uint64_t orc_write_value(uint64_t seed, uint64_t tx, uint32_t op);
+void orc_seed_table(uint64_t seed, uint64_t rows, uint32_t cols, uint64_t* cells);
+
+/* ---- workload generator restatement (generation/gen_ycsb.cu:76-150) ---- */
+double orc_zeta(uint64_t n, double theta);
+uint64_t orc_zipf_rank(uint64_t n, double theta, double zetan, double u);
+/* Ops of transaction `tx` (global id); returns the op count. */
+uint32_t orc_gen_tx(uint64_t rows, uint32_t cols, uint32_t len_min, uint32_t len_max, double theta,
+                    double read_ratio, uint64_t seed, double zetan, uint64_t tx, uint64_t* out);
+
+/* ---- sorted-array index: SortedArray_GPU::Find (include/index.cuh:74-91) ---- */
+uint64_t orc_find(const uint32_t* sorted_keys, const uint64_t* vals, uint32_t length, uint32_t key);
+
+/* ---- serial replay ----
+ * Executes the transactions one after another in ascending (batch, key0, key1)
+ * of order[3*t .. 3*t+2] (ties broken by transaction id and reported), each op
+ * of a transaction in op order: a read returns the cell's current value, a
+ * write stores orc_write_value(seed, t, j).  Keys map to rows through the
+ * identity index (key k -> row k, keys >= rows are skipped).
+ * table: rows*cols cells, updated in place; reads: one per op (writes get 0).
+ * Returns the number of tied order keys (0 for a strict order), or -1 on
+ * allocation failure. */
+int64_t orc_replay(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                   const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads);
+/* Replay in plain transaction-id order (GaccO's serial order inside a batch). */
+void orc_replay_txid(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                     const uint64_t* tx_off, uint64_t ntx, uint64_t* reads);
+
+/* ---- GPUTx one-pass rank (src/gputx.cu:5-38, include/gputx.cuh:213-294) ----
+ * rank: one per transaction; counts: per-rank transaction counts (length
+ * returned in *nranks, caller provides ntx+1 slots).  Keys >= rows are
+ * skipped. */
+int orc_gputx_rank(uint64_t rows, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx,
+                   uint32_t* rank, uint32_t* counts, uint64_t* nranks);
+
+/* ---- conflict-graph serializability check (src/runtime.cu:23-98) ----
+ * Given the committed serial order and the read values, checks that every read
+ * returned the value of the latest preceding write in that order (the
+ * "reads-from" relation) -- equivalent to the replay check but reported per
+ * op.  Returns the number of mismatching reads. */
+uint64_t orc_check_reads(uint64_t rows, uint32_t cols, uint64_t seed, const uint64_t* init_table,
+                         const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
+                         const uint64_t* reads);
+
+/* ---- multi-threaded CPU path (the CPU baseline): the same protocols on C11
+ * atomics, one transaction per thread at a time, ops in key order.
+ * scheme: 0 tpl_nw, 1 tpl_wd, 2 to, 3 mvcc, 4 silo, 5 tictoc, 6 gacco, 7 gputx.
+ * Fills order (3 u64 per tx) and reads like the GPU path; returns 0 or <0. */
+typedef struct orc_cpu_stats {
+  double seconds;
+  uint64_t aborts;
+  uint64_t committed;
+  int threads;
+} orc_cpu_stats;
+int orc_cpu_run(int scheme, uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table,
+                const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, uint64_t batch_size, int threads,
+                uint64_t* order, uint64_t* reads, orc_cpu_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,261 @@
+/*
+ * oracle_core.c -- CPU restatement of the YCSB path's arithmetic.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define TAG_KEY 0x4b45590000000000ull
+#define TAG_RW 0x5257000000000000ull
+#define TAG_COL 0x434f4c0000000000ull
+#define TAG_WRITE 0x5752495445000000ull
+#define TAG_LEN 0x4c454e0000000000ull
+
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+uint64_t orc_splitmix64(uint64_t x) { return mix64(x + 0x9E3779B97F4A7C15ull); }
+uint64_t orc_hash3(uint64_t seed, uint64_t a, uint64_t b) {
+  return orc_splitmix64(orc_splitmix64(seed ^ a) ^ b);
+}
+uint64_t orc_cell_init(uint64_t seed, uint64_t row, uint32_t col, uint32_t cols) {
+  return orc_splitmix64(seed ^ (row * (uint64_t)cols + col));
+}
+uint64_t orc_write_value(uint64_t seed, uint64_t tx, uint32_t op) {
+  return orc_hash3(seed ^ TAG_WRITE, tx, op);
+}
+void orc_seed_table(uint64_t seed, uint64_t rows, uint32_t cols, uint64_t* cells) {
+  for (uint64_t r = 0; r < rows; ++r)
+    for (uint32_t c = 0; c < cols; ++c) cells[r * cols + c] = orc_cell_init(seed, r, c, cols);
+}
+
+/* ---- generator (generation/gen_ycsb.cu) ---- */
+static double unit_double(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+/* zeta(n, theta), gen_ycsb.cu:76-82, summed in blocks of 65536 terms */
+double orc_zeta(uint64_t n, double theta) {
+  double total = 0.0;
+  for (uint64_t lo = 1; lo <= n; lo += 65536) {
+    uint64_t hi = lo + 65535;
+    if (hi > n) hi = n;
+    double s = 0.0;
+    for (uint64_t i = lo; i <= hi; ++i) s += pow(1.0 / (double)i, theta);
+    total += s;
+  }
+  return total;
+}
+
+/* zipf(), gen_ycsb.cu:84-102 with n = rows */
+uint64_t orc_zipf_rank(uint64_t n, double theta, double zetan, double u) {
+  const double zeta2 = orc_zeta(2, theta);
+  const double alpha = 1.0 / (1.0 - theta);
+  const double eta = (1.0 - pow(2.0 / (double)n, 1.0 - theta)) / (1.0 - zeta2 / zetan);
+  const double uz = u * zetan;
+  if (uz < 1.0) return 1;
+  if (uz < 1.0 + pow(0.5, theta)) return n >= 2 ? 2 : 1;
+  uint64_t r = 1 + (uint64_t)((double)n * pow(eta * u - eta + 1.0, alpha));
+  if (r > n) r = n;
+  return r;
+}
+
+static int cmp_key(const void* a, const void* b) {
+  const uint32_t x = (uint32_t)(*(const uint64_t*)a), y = (uint32_t)(*(const uint64_t*)b);
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+uint32_t orc_gen_tx(uint64_t rows, uint32_t cols, uint32_t len_min, uint32_t len_max, double theta,
+                    double read_ratio, uint64_t seed, double zetan, uint64_t tx, uint64_t* out) {
+  uint32_t L = len_min;
+  if (len_max > len_min) L = len_min + (uint32_t)(orc_hash3(seed ^ TAG_LEN, tx, 0) % (uint64_t)(len_max - len_min + 1));
+  uint32_t keys[64];
+  uint64_t d = 0;
+  const uint64_t max_draws = 64ull * L + 64;
+  for (uint32_t s = 0; s < L; ++s) {
+    int ok = 0;
+    uint32_t key = 0;
+    while (d < max_draws) {
+      const uint64_t row = orc_zipf_rank(rows, theta, zetan, unit_double(orc_hash3(seed ^ TAG_KEY, tx, d++))) - 1;
+      int dup = 0;
+      for (uint32_t q = 0; q < s; ++q) dup |= keys[q] == (uint32_t)row;
+      if (!dup) {
+        key = (uint32_t)row;
+        ok = 1;
+        break;
+      }
+    }
+    if (!ok) {
+      for (uint32_t cand = 0;; ++cand) {
+        int dup = 0;
+        for (uint32_t q = 0; q < s; ++q) dup |= keys[q] == cand;
+        if (!dup) {
+          key = cand;
+          break;
+        }
+      }
+    }
+    keys[s] = key;
+    const int is_write = !(unit_double(orc_hash3(seed ^ TAG_RW, tx, s)) < read_ratio);
+    const uint32_t col = (uint32_t)(orc_hash3(seed ^ TAG_COL, tx, s) % cols);
+    out[s] = (uint64_t)key | ((uint64_t)(col & 0xff) << 32) | ((uint64_t)is_write << 40);
+  }
+  qsort(out, L, sizeof(uint64_t), cmp_key);
+  return L;
+}
+
+/* ---- SortedArray_GPU::Find, include/index.cuh:74-91 ---- */
+uint64_t orc_find(const uint32_t* keys, const uint64_t* vals, uint32_t length, uint32_t key) {
+  uint32_t st = 0, en = length;
+  while (st < en) {
+    const uint32_t mid = (st + en) >> 1;
+    const uint32_t k = keys[mid];
+    if (k == key) return vals[mid];
+    if (k > key)
+      en = mid;
+    else
+      st = mid + 1;
+  }
+  if (st < length && keys[st] == key) return vals[st];
+  return length;
+}
+
+/* ---- serial replay ---- */
+typedef struct {
+  uint64_t b, k0, k1, tx;
+} okey_t;
+static int cmp_okey(const void* pa, const void* pb) {
+  const okey_t* a = (const okey_t*)pa;
+  const okey_t* b = (const okey_t*)pb;
+  if (a->b != b->b) return a->b < b->b ? -1 : 1;
+  if (a->k0 != b->k0) return a->k0 < b->k0 ? -1 : 1;
+  if (a->k1 != b->k1) return a->k1 < b->k1 ? -1 : 1;
+  if (a->tx != b->tx) return a->tx < b->tx ? -1 : 1;
+  return 0;
+}
+
+static void apply_tx(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                     const uint64_t* tx_off, uint64_t t, uint64_t* reads) {
+  for (uint64_t i = tx_off[t]; i < tx_off[t + 1]; ++i) {
+    const uint64_t op = ops[i];
+    const uint64_t row = (uint32_t)op;
+    const uint32_t col = (uint32_t)(op >> 32) & 0xff;
+    if (row >= rows || col >= cols) {
+      if (reads) reads[i] = 0;
+      continue;
+    }
+    uint64_t* cell = table + row * cols + col;
+    if ((op >> 40) & 1) {
+      *cell = orc_write_value(seed, t, (uint32_t)(i - tx_off[t]));
+      if (reads) reads[i] = 0;
+    } else if (reads) {
+      reads[i] = *cell;
+    }
+  }
+}
+
+int64_t orc_replay(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                   const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads) {
+  okey_t* ord = (okey_t*)malloc(sizeof(okey_t) * (ntx ? ntx : 1));
+  if (!ord) return -1;
+  for (uint64_t t = 0; t < ntx; ++t) {
+    ord[t].b = order[3 * t];
+    ord[t].k0 = order[3 * t + 1];
+    ord[t].k1 = order[3 * t + 2];
+    ord[t].tx = t;
+  }
+  qsort(ord, ntx, sizeof(okey_t), cmp_okey);
+  int64_t ties = 0;
+  for (uint64_t i = 1; i < ntx; ++i)
+    if (ord[i].b == ord[i - 1].b && ord[i].k0 == ord[i - 1].k0 && ord[i].k1 == ord[i - 1].k1) ties++;
+  for (uint64_t i = 0; i < ntx; ++i) apply_tx(rows, cols, seed, table, ops, tx_off, ord[i].tx, reads);
+  free(ord);
+  return ties;
+}
+
+void orc_replay_txid(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                     const uint64_t* tx_off, uint64_t ntx, uint64_t* reads) {
+  for (uint64_t t = 0; t < ntx; ++t) apply_tx(rows, cols, seed, table, ops, tx_off, t, reads);
+}
+
+uint64_t orc_check_reads(uint64_t rows, uint32_t cols, uint64_t seed, const uint64_t* init_table,
+                         const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
+                         const uint64_t* reads) {
+  uint64_t* table = (uint64_t*)malloc(sizeof(uint64_t) * rows * cols);
+  uint64_t* exp = (uint64_t*)malloc(sizeof(uint64_t) * (tx_off[ntx] ? tx_off[ntx] : 1));
+  if (!table || !exp) {
+    free(table);
+    free(exp);
+    return ~0ull;
+  }
+  memcpy(table, init_table, sizeof(uint64_t) * rows * cols);
+  orc_replay(rows, cols, seed, table, ops, tx_off, ntx, order, exp);
+  uint64_t bad = 0;
+  for (uint64_t i = 0; i < tx_off[ntx]; ++i)
+    if (!((ops[i] >> 40) & 1) && exp[i] != reads[i]) bad++;
+  free(table);
+  free(exp);
+  return bad;
+}
+
+/* ---- GPUTx one-pass rank ----
+ * Accesses sorted by (row, tx) (include/gputx.cuh:266, GTX_AccessItem order);
+ * per row, cur_rank starts at 0, every access raises rank[tx] to cur_rank, and
+ * cur_rank grows by one between two neighbours of which one writes
+ * (src/gputx.cu:17-27).  The reference reads the end of the last row's segment
+ * past the scanned offsets; here every segment ends where the next row starts
+ * (the last one at the op count).  counts = histogram of ranks
+ * (src/gputx.cu:30-38); nranks = max local rank + 1 (gputx.cuh:283-294). */
+typedef struct {
+  uint32_t row, tx;
+  uint8_t write;
+} acc_t;
+static int cmp_acc(const void* pa, const void* pb) {
+  const acc_t* a = (const acc_t*)pa;
+  const acc_t* b = (const acc_t*)pb;
+  if (a->row != b->row) return a->row < b->row ? -1 : 1;
+  if (a->tx != b->tx) return a->tx < b->tx ? -1 : 1;
+  return 0;
+}
+int orc_gputx_rank(uint64_t rows, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx,
+                   uint32_t* rank, uint32_t* counts, uint64_t* nranks) {
+  const uint64_t nops = tx_off[ntx];
+  acc_t* acc = (acc_t*)malloc(sizeof(acc_t) * (nops ? nops : 1));
+  if (!acc) return -1;
+  uint64_t n = 0;
+  for (uint64_t t = 0; t < ntx; ++t)
+    for (uint64_t i = tx_off[t]; i < tx_off[t + 1]; ++i) {
+      const uint32_t row = (uint32_t)ops[i];
+      if (row >= rows) continue;
+      acc[n].row = row;
+      acc[n].tx = (uint32_t)t;
+      acc[n].write = (uint8_t)((ops[i] >> 40) & 1);
+      n++;
+    }
+  qsort(acc, n, sizeof(acc_t), cmp_acc);
+  for (uint64_t t = 0; t < ntx; ++t) rank[t] = 0;
+  uint32_t tot = 0;
+  uint64_t st = 0;
+  while (st < n) {
+    uint64_t en = st;
+    while (en < n && acc[en].row == acc[st].row) en++;
+    uint32_t cur = 0;
+    for (uint64_t i = st; i < en; ++i) {
+      if (rank[acc[i].tx] < cur) rank[acc[i].tx] = cur;
+      if (i + 1 < en && (acc[i].write || acc[i + 1].write)) cur++;
+    }
+    if (cur + 1 > tot) tot = cur + 1;
+    st = en;
+  }
+  free(acc);
+  if (n == 0 && ntx) tot = 1;
+  if (nranks) *nranks = tot;
+  if (counts) {
+    for (uint32_t r = 0; r < tot; ++r) counts[r] = 0;
+    for (uint64_t t = 0; t < ntx; ++t) counts[rank[t]]++;
+  }
+  return 0;
+}
