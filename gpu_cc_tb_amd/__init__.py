@@ -100,6 +100,8 @@ def load_library() -> C.CDLL:
                                           C.POINTER(C.c_uint64)]),
         "gcctb_write_txs_file": (C.c_int, [C.c_char_p, _u64p, _u64p, C.c_uint64]),
         "gcctb_device_info": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+        "gcctb_microbench": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -194,6 +196,18 @@ def device_info() -> dict:
     cus = C.c_int32(0)
     _check(lib.gcctb_device_info(name, 256, C.byref(khz), C.byref(cus)))
     return {"name": name.value.decode(), "wall_clock_khz": khz.value, "cus": cus.value}
+
+
+MICROBENCH = {"cas_random": 0, "add_random": 1, "add_same": 2, "load_coherent": 3, "load_cached": 4, "copy": 5}
+
+
+def microbench(kind: str, n_words: int, threads: int = 1 << 20, iters: int = 16) -> dict:
+    """Memory-system ceiling: rate in ops/s (bytes/s for 'copy')."""
+    lib = load_library()
+    rate, ms = C.c_double(0), C.c_double(0)
+    _check(lib.gcctb_microbench(MICROBENCH[kind], n_words, threads, iters, C.byref(rate), C.byref(ms)))
+    return {"kind": kind, "n_words": n_words, "threads": threads, "iters": iters, "rate": rate.value,
+            "ms": ms.value}
 
 
 # ---------------------------------------------------------------------------

@@ -36,6 +36,7 @@ DEVICE_SRCS = [
     "device/exec_silo.hip",
     "device/exec_tictoc.hip",
     "device/exec_ordered.hip",
+    "device/microbench.hip",
 ]
 HOST_SRCS = ["host/engine.cpp", "host/workload.cpp"]
 TOOLS = {"test_ycsb": "tools/test_ycsb.cpp", "gen_ycsb": "tools/gen_ycsb.cpp"}

@@ -70,16 +70,26 @@ __device__ __forceinline__ uint32_t op_index(int k) {
   return (lane_id() & (G - 1)) + (uint32_t)k * G;
 }
 
-// Abort bookkeeping, identical on every lane of the group.
+// Abort bookkeeping, identical on every lane of the group (the wall clock is a
+// scalar read, so every lane of the wave sees the same value).
+//
+// Retry backoff.  The lanes of a wave attempt in lockstep, so two conflicting
+// transactions of one wave that restart in the same round collide again in the
+// same order; an immediate retry (benchmark/ycsb.cu:99-104) can then repeat
+// forever.  The restart is delayed by a random number of units in
+// [0, 2^min(attempts, 10)), where a unit is the duration of the failed attempt
+// (at least backoff_ticks): retries spread over rounds, not just over ticks.
 template <int G>
 __device__ __forceinline__ void abort_attempt(TxCtx<G>& c, const ExecArgs& a, uint64_t now) {
+  (void)now;
+  const uint64_t t = now_ticks();
+  const uint64_t dur = t - c.t_attempt;
   c.attempts++;
-  c.m_abort += now - c.t_attempt;
-  if (a.backoff_ticks) {
-    const uint64_t h = hash3(a.seed ^ TAG_BACKOFF, c.tx, c.attempts);
-    const uint32_t span = 1u << (c.attempts < 6 ? c.attempts : 6);
-    c.t_resume = now + (h % span) * a.backoff_ticks;
-  }
+  c.m_abort += dur;
+  const uint64_t unit = dur > a.backoff_ticks ? dur : a.backoff_ticks;
+  const uint64_t h = hash3(a.seed ^ TAG_BACKOFF, c.tx, c.attempts);
+  const uint32_t span = 1u << (c.attempts < 10 ? c.attempts : 10);
+  c.t_resume = t + (h % span) * unit;
   c.ph = PH_START;
 }
 
@@ -868,11 +878,29 @@ __global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
   c.m_idx = now_ticks() - c.t_start;
 #pragma unroll 1
   while (ballot(!c.done) != 0) {
-    S::round(c, a);
+    // every live group of the wave is backing off: sleep instead of spinning
+    if (ballot(!c.done && !(c.ph == PH_START && c.t_round < c.t_resume)) == 0)
+      __builtin_amdgcn_s_sleep(4);
+    else
+      S::round(c, a);
     c.t_round = now_ticks();
     // Watchdog: the wall clock is a scalar read, so the exit is wave-uniform.
     if (a.watchdog_ticks && c.t_round - c.t_start > a.watchdog_ticks) {
       if (lane_id() == 0) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_WATCHDOG);
+      if (a.dbg && !c.done) {
+        // per op of a stopped transaction: flags/row, the CC word through the
+        // coherent load and through an atomic read, and phase/attempts
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (!(c.meta[k] & M_VALID)) continue;
+          uint64_t* d = a.dbg + (c.op0 + op_index<G>(k)) * 4;
+          uint64_t* w = a.cc + c.row[k];
+          d[0] = ((uint64_t)c.meta[k] << 32) | c.row[k];
+          d[1] = ld(w);
+          d[2] = fetch_or(w, 0);
+          d[3] = ((uint64_t)c.ph << 56) | ((uint64_t)c.attempts << 24) | 1u;
+        }
+      }
       break;
     }
   }
@@ -883,18 +911,23 @@ __global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
                    leader ? c.m_tot : 0ull,   leader ? c.m_idx : 0ull, leader ? c.m_commit : 0ull};
   misses = group_sum64<G>(misses);
   const uint64_t miss_leader = leader ? misses : 0ull;
+  // A block of block_size*32 threads can end in a half wave (lanes 0..31);
+  // the last butterfly step then has no partner lanes and is skipped.
+  const bool full_wave = (ballot(true) >> 32) != 0;
 #pragma unroll
   for (int f = 0; f < 8; ++f) {
     uint64_t x = v[f];
 #pragma unroll
-    for (int m = 1; m < 64; m <<= 1) x += shfl_xor64(x, m);
+    for (int m = 1; m < 32; m <<= 1) x += shfl_xor64(x, m);
+    if (full_wave) x += shfl_xor64(x, 32);
     v[f] = x;
   }
   uint64_t mm = miss_leader;
 #pragma unroll
-  for (int m = 1; m < 64; m <<= 1) mm += shfl_xor64(mm, m);
+  for (int m = 1; m < 32; m <<= 1) mm += shfl_xor64(mm, m);
+  if (full_wave) mm += shfl_xor64(mm, 32);
   if (lane_id() == 0) {
-    unsigned long long* M = (unsigned long long*)a.metrics;
+    unsigned long long* M = (unsigned long long*)(a.metrics + (blockIdx.x & (METRIC_SLOTS - 1)));
     if (v[0]) atomicAdd(M + 0, v[0]);
     atomicAdd(M + 1, v[1]);
     atomicAdd(M + 2, v[2]);

@@ -77,6 +77,10 @@ hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint
 hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s);
 hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t* hist, hipStream_t s);
 
+// --- memory-system ceilings (microbench.hip) ---
+hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint64_t iters, double* rate,
+                      double* ms_out);
+
 // --- per-batch state ---
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
                       uint32_t* vprev, hipStream_t s);

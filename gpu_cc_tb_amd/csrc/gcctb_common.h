@@ -147,6 +147,10 @@ struct Metrics {
   uint64_t pad[6];
 };
 constexpr int METRICS_WORDS = 16;
+// Waves add their totals into one of METRIC_SLOTS 128-byte slots (by block
+// index) so that the end-of-kernel atomics do not pile onto one address; the
+// host sums the slots.
+constexpr int METRIC_SLOTS = 256;
 
 // error_flags bits
 constexpr uint64_t ERR_TS_OVERFLOW = 1;     // 31-bit timestamp space exhausted
@@ -208,6 +212,7 @@ struct ExecArgs {
   uint32_t backoff_ticks;  // randomized backoff scale after an abort (0 = none)
   const uint32_t* tx_rank; // gputx: per-tx rank (okey0)
   uint64_t watchdog_ticks; // a wave gives up after this many ticks (0 = never)
+  uint64_t* dbg;           // optional: 4 u64 per op, written for transactions the watchdog stopped
 };
 
 }  // namespace gcctb

@@ -114,6 +114,9 @@ struct gcctb_ctx {
   gcctb::DevBuf<uint64_t> sk64a, sk64b;
   gcctb::DevBuf<uint32_t> rank, tx_map, max_rank;
   gcctb::DevBuf<uint64_t> conflicts;
+  // watchdog diagnostics (GCCTB_DEBUG_DUMP=<file>)
+  gcctb::DevBuf<uint64_t> dbg;
+  std::string dbg_path;
   uint64_t nranks = 0;
   std::vector<uint32_t> h_rank_counts;
   std::unique_ptr<gcctb::SchemeHost> host;
@@ -185,6 +188,7 @@ class SchemeHost {
     a.backoff_ticks = ctx.cfg.backoff_ticks;
     a.tx_rank = nullptr;
     a.watchdog_ticks = ctx.watchdog_ticks;
+    a.dbg = ctx.dbg.p;
     return a;
   }
   gcctb_ctx& ctx;
@@ -394,7 +398,7 @@ void ensure_state(gcctb_ctx& c) {
   }
   c.ctrs.alloc(8);
   HIP_TRY(hipMemsetAsync(c.ctrs.p, 0, 64, c.stream));
-  c.metrics.alloc(1);
+  c.metrics.alloc(METRIC_SLOTS);
 }
 
 }  // namespace
@@ -432,6 +436,8 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
     const char* wd = std::getenv("GCCTB_WATCHDOG_S");
     const double sec = wd ? std::atof(wd) : 20.0;
     c->watchdog_ticks = sec > 0 ? (uint64_t)(sec * 1e8) : 0;
+    const char* dump = std::getenv("GCCTB_DEBUG_DUMP");
+    if (dump && *dump) c->dbg_path = dump;
   }
   try {
     HIP_TRY(hipSetDevice(0));
@@ -603,10 +609,14 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
     c->okey0.alloc(ntx);
     c->okey1.alloc(ntx);
     c->tx_aborts.alloc(ntx);
+    if (!c->dbg_path.empty()) {
+      c->dbg.alloc(nops * 4);
+      HIP_TRY(hipMemsetAsync(c->dbg.p, 0, c->dbg.n * sizeof(uint64_t), c->stream));
+    }
     if (nops) HIP_TRY(hipMemcpyAsync(c->ops.p, ops, nops * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tx_off.p, tx_off, (ntx + 1) * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemsetAsync(c->reads.p, 0, nops * 8 + 8, c->stream));
-    HIP_TRY(hipMemsetAsync(c->tx_aborts.p, 0, ntx * 4 + 4, c->stream));
+    HIP_TRY(hipMemsetAsync(c->reads.p, 0, c->reads.n * sizeof(uint64_t), c->stream));
+    HIP_TRY(hipMemsetAsync(c->tx_aborts.p, 0, c->tx_aborts.n * sizeof(uint32_t), c->stream));
     HIP_TRY(launch::op_tx(c->tx_off.p, ntx, c->op_tx.p, c->stream));
     if (c->scheme == SCHEME_MVCC) {
       c->pool_cap = cap;
@@ -656,7 +666,7 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
   Metrics m{};
   try {
     hipStream_t s = c->stream;
-    HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics), s));
+    HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics) * METRIC_SLOTS, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0;
     double pre = 0, proc = 0;
@@ -697,7 +707,22 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
     st.proc_ms = proc;
     st.exec_kernel_ms = proc;
     st.batches = launches;
-    HIP_TRY(hipMemcpy(&m, c->metrics.p, sizeof(Metrics), hipMemcpyDeviceToHost));
+    {
+      std::vector<Metrics> slots(METRIC_SLOTS);
+      HIP_TRY(hipMemcpy(slots.data(), c->metrics.p, sizeof(Metrics) * METRIC_SLOTS, hipMemcpyDeviceToHost));
+      for (const Metrics& x : slots) {
+        m.abort += x.abort;
+        m.ts_duration += x.ts_duration;
+        m.wait_duration += x.wait_duration;
+        m.abort_duration += x.abort_duration;
+        m.manager_duration += x.manager_duration;
+        m.tot_duration += x.tot_duration;
+        m.index_duration += x.index_duration;
+        m.committed += x.committed;
+        m.index_miss += x.index_miss;
+        m.error_flags |= x.error_flags;
+      }
+    }
     if (c->scheme == SCHEME_GPUTX) {
       st.ranks = c->nranks;
       if (c->nops) HIP_TRY(hipMemcpy(&st.same_rank_conflicts, c->conflicts.p, 8, hipMemcpyDeviceToHost));
@@ -705,6 +730,24 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
     int rate = 0;
     HIP_TRY(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0));
     st.tick_khz = rate;
+    if ((m.error_flags & ERR_WATCHDOG) && c->dbg.p) {
+      std::vector<uint64_t> d(c->dbg.n);
+      HIP_TRY(hipMemcpy(d.data(), c->dbg.p, d.size() * 8, hipMemcpyDeviceToHost));
+      if (FILE* f = std::fopen(c->dbg_path.c_str(), "a")) {
+        std::fprintf(f, "# op tx meta row ld_word atomic_word phase attempts\n");
+        for (uint64_t t = 0; t < c->ntx; ++t)
+          for (uint64_t i = c->h_tx_off[t]; i < c->h_tx_off[t + 1]; ++i) {
+            const uint64_t* e = &d[i * 4];
+            if (!(e[3] & 1)) continue;
+            std::fprintf(f, "%llu %llu %llx %llu %llx %llx %llu %llu\n", (unsigned long long)i,
+                         (unsigned long long)t, (unsigned long long)(e[0] >> 32),
+                         (unsigned long long)(e[0] & 0xffffffffu), (unsigned long long)e[1],
+                         (unsigned long long)e[2], (unsigned long long)(e[3] >> 56),
+                         (unsigned long long)((e[3] >> 24) & 0xffffffffu));
+          }
+        std::fclose(f);
+      }
+    }
   } catch (const HipFailure& f) {
     return hip_fail(f);
   }
@@ -722,7 +765,15 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
   st.total_ops = c->nops;
   st.read_ops = c->nreads;
   if (out) *out = st;
-  if (m.error_flags) return set_error(GCCTB_EKERNEL, "kernel error flags 0x" + std::to_string(m.error_flags));
+  if (m.error_flags) {
+    std::string what;
+    if (m.error_flags & ERR_TS_OVERFLOW) what += " timestamp-overflow";
+    if (m.error_flags & ERR_POOL_OVERFLOW) what += " version-pool-full";
+    if (m.error_flags & ERR_CHAIN_BROKEN) what += " version-chain-broken";
+    if (m.error_flags & ERR_TXLEN) what += " transaction-too-long";
+    if (m.error_flags & ERR_WATCHDOG) what += " watchdog";
+    return set_error(GCCTB_EKERNEL, "kernel error flags " + std::to_string(m.error_flags) + ":" + what);
+  }
   if (m.committed != c->ntx) return set_error(GCCTB_EKERNEL, "not every transaction committed");
   return GCCTB_OK;
 }
@@ -807,6 +858,16 @@ int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int
   (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0);
   if (wall_clock_khz) *wall_clock_khz = rate;
   if (cus) *cus = p.multiProcessorCount;
+  return GCCTB_OK;
+}
+
+int gcctb_microbench(int32_t which, uint64_t n_words, uint64_t threads, uint64_t iters, double* rate, double* ms) {
+  if (which < 0 || which > 5 || n_words < 2 || threads == 0 || iters == 0)
+    return set_error(GCCTB_EINVAL, "bad microbench arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(GCCTB_EDEVICE, "no HIP device");
+  const hipError_t e = launch::microbench(which, n_words, threads, iters, rate, ms);
+  if (e != hipSuccess) return set_error(GCCTB_EDEVICE, std::string("microbench: ") + hipGetErrorString(e));
   return GCCTB_OK;
 }
 

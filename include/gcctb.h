@@ -41,7 +41,8 @@ typedef struct gcctb_config {
   uint64_t rows;           /* table rows (<= 2^32 - 2) */
   uint32_t cols;           /* u64 columns per row (1..255) */
   uint64_t seed;           /* seeds the table and the written values */
-  uint32_t backoff_ticks;  /* randomized retry backoff scale (10 ns ticks), 0 = immediate retry */
+  uint32_t backoff_ticks;  /* minimum unit of the randomized retry backoff (10 ns ticks); the unit
+                              is the failed attempt's duration when that is longer */
   uint32_t reserved;
 } gcctb_config;
 
@@ -150,6 +151,17 @@ int gcctb_write_txs_file(const char* path, const uint64_t* ops, const uint64_t* 
 
 /* Device information. */
 int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int32_t* cus);
+
+/* Memory-system ceilings of the GPU for the roofline (no reference
+ * counterpart; measured with the kernels' own access forms):
+ *   which 0: random 64-bit agent-scope CAS over n_words words
+ *         1: random 64-bit agent-scope atomic add
+ *         2: atomic add to one word (one lane per wave)
+ *         3: random coherent (agent-scope) 8-byte load
+ *         4: random cached 8-byte load
+ *         5: streaming copy of n_words words (rate in bytes/s, read + write)
+ * threads x iters operations; *rate in operations/s, *ms the timed launch. */
+int gcctb_microbench(int32_t which, uint64_t n_words, uint64_t threads, uint64_t iters, double* rate, double* ms);
 
 #ifdef __cplusplus
 }

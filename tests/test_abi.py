@@ -1,0 +1,66 @@
+"""The C-ABI library loads and exports every entry point of include/gcctb.h;
+host-only entry points work without a GPU; GPU entry points fail loudly."""
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+import gpu_cc_tb_amd as g
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "gcctb.h")).read()
+    return sorted(set(re.findall(r"\b(gcctb_[a-z_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    lib = g.load_library()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_bad_config_is_rejected_before_device_use():
+    for kw in [dict(warp_density=6), dict(block_size=33), dict(block_size=0), dict(batch_size=0)]:
+        with pytest.raises(g.GcctbError) as e:
+            g.Context("tpl_nw", rows=16, **kw)
+        assert e.value.code == -1
+    with pytest.raises(g.GcctbError) as e:
+        g.Context("slow_silo", rows=16)
+    assert e.value.code == -1
+
+
+def test_no_cpu_fallback_without_gpu():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    with pytest.raises(g.GcctbError) as e:
+        g.Context("tpl_nw", rows=16)
+    assert e.value.code == -3
+
+
+def test_txs_file_round_trip():
+    w = g.generate(1000, 300, theta=0.8, read_ratio=0.5, len_min=1, len_max=16)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "x.txs")
+        g.write_txs(p, w)
+        assert os.path.getsize(p) == 136 * 300
+        r = g.read_txs(p, 300)
+    assert np.array_equal(r.tx_off, w.tx_off)
+    assert np.array_equal(g.op_key(r.ops), g.op_key(w.ops))
+    assert np.array_equal(g.op_is_write(r.ops), g.op_is_write(w.ops))
+    assert np.all(g.op_col(r.ops) < 10)
+
+
+def test_generator_rejects_bad_input():
+    for kw in [dict(theta=1.0), dict(theta=-0.1), dict(read_ratio=1.5), dict(len_min=0), dict(len_min=33)]:
+        args = dict(theta=0.5, read_ratio=0.5)
+        args.update(kw)
+        with pytest.raises(g.GcctbError):
+            g.generate(100, 10, **args)
+    with pytest.raises(g.GcctbError):
+        g.generate(4, 10, theta=0.5, read_ratio=0.5, len_min=8)
