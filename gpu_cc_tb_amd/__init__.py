@@ -100,6 +100,7 @@ def load_library() -> C.CDLL:
                                           C.POINTER(C.c_uint64)]),
         "gcctb_write_txs_file": (C.c_int, [C.c_char_p, _u64p, _u64p, C.c_uint64]),
         "gcctb_device_info": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+        "gcctb_check": (C.c_int, [P, _u64p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "gcctb_microbench": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]),
     }
@@ -301,6 +302,13 @@ class Context:
         counts = np.zeros(max(1, n.value), dtype=np.uint32)
         _check(self._lib.gcctb_fetch_ranks(self._h, None, _ptr(counts), C.byref(n)))
         return rank[: self.ntx], counts[: n.value]
+
+    def check(self, init_table: np.ndarray) -> tuple[int, int]:
+        """Debug-mode serializability check of the last run: (bad, uncommitted)."""
+        bad, unc = C.c_uint64(0), C.c_uint64(0)
+        _check(self._lib.gcctb_check(self._h, np.ascontiguousarray(init_table, dtype=np.uint64), C.byref(bad),
+                                     C.byref(unc)))
+        return bad.value, unc.value
 
     def fetch_cc(self) -> np.ndarray:
         w = np.zeros(self.rows, dtype=np.uint64)

@@ -38,7 +38,7 @@ DEVICE_SRCS = [
     "device/exec_ordered.hip",
     "device/microbench.hip",
 ]
-HOST_SRCS = ["host/engine.cpp", "host/workload.cpp"]
+HOST_SRCS = ["host/engine.cpp", "host/workload.cpp", "host/check.cpp"]
 TOOLS = {"test_ycsb": "tools/test_ycsb.cpp", "gen_ycsb": "tools/gen_ycsb.cpp"}
 ORACLE_SRCS = ["oracle_core.c", "cpu_cc.c"]
 

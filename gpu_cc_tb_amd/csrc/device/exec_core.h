@@ -120,6 +120,17 @@ __device__ __forceinline__ bool backoff_over(const TxCtx<G>& c, uint64_t now) {
   return now >= c.t_resume;
 }
 
+// Timestamp for a starting attempt (NaiveTSAllocator::Alloc,
+// include/timestamp.cuh:22-25): wave-aggregated, from the block's shard.
+// Shard s of TS_SHARDS issues s, s + TS_SHARDS, ...; the allocators restart at
+// 1 every batch (timestamp.cuh:76-80), so the smallest timestamp is TS_SHARDS.
+template <int G>
+__device__ __forceinline__ uint64_t alloc_ts(const ExecArgs& a, bool start) {
+  const uint32_t shard = blockIdx.x & (TS_SHARDS - 1);
+  const uint64_t local = wave_fetch_inc(a.ctrs + CTR_TS0 + 16 * shard, start && (lane_id() & (G - 1)) == 0);
+  return (local << TS_SHARD_BITS) | shard;
+}
+
 // ===========================================================================
 // Two-phase locking, no-wait and wait-die (include/twopl.cuh:59-382).
 // Strict 2PL: shared locks for reads, exclusive locks for writes, writes are
@@ -129,37 +140,34 @@ __device__ __forceinline__ bool backoff_over(const TxCtx<G>& c, uint64_t now) {
 // ===========================================================================
 template <bool WAIT_DIE>
 struct TwoPL {
-  // lock_shared (twopl.cuh:233-267)
+  // lock_shared (twopl.cuh:233-267).  The first CAS guesses a free word (0),
+  // so an uncontended lock costs one atomic and no load.
   static __device__ StepResult lock_shared(uint64_t* w, uint64_t self) {
-    uint64_t cur = ld(w);
+    uint64_t cur = 0;
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
-      uint64_t nxt;
-      if (tpl::cnt(cur) == 0) {
-        nxt = tpl::make(1, 1, tpl::holder(cur));
-      } else if (tpl::shared(cur)) {
-        nxt = cur + (1ull << 1);
-      } else {
+      if (tpl::excl(cur)) {
         const uint64_t h = tpl::holder(cur);
         if (h == self) return R_OK;
         if (!WAIT_DIE) return R_ABORT;
         if (h < self) return R_ABORT;  // die: an older transaction holds it
         return R_WAIT;                 // wait for a younger holder
       }
-      if (cas(w, cur, nxt)) return R_OK;
+      if (cas(w, cur, cur + tpl::CNT_ONE)) return R_OK;
     }
     return R_WAIT;
   }
-  // lock_exclusive (twopl.cuh:269-303)
+  // lock_exclusive (twopl.cuh:269-303); a shared-held word always aborts
+  // (twopl.cuh:285-286).
   static __device__ StepResult lock_excl(uint64_t* w, uint64_t self) {
-    uint64_t cur = ld(w);
+    uint64_t cur = 0;
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
-      if (tpl::cnt(cur) == 0) {
-        if (cas(w, cur, tpl::make(0, 1, self))) return R_OK;
+      if (cur == 0) {
+        if (cas(w, cur, tpl::make_excl(self))) return R_OK;
         continue;
       }
-      if (tpl::shared(cur)) return R_ABORT;
+      if (!tpl::excl(cur)) return R_ABORT;
       const uint64_t h = tpl::holder(cur);
       if (h == self) return R_OK;
       if (!WAIT_DIE) return R_ABORT;
@@ -168,13 +176,13 @@ struct TwoPL {
     }
     return R_WAIT;
   }
-  // unlock_shared: holder_cnt-- (twopl.cuh:305-315); unlock_exclusive clears
-  // holder_cnt (twopl.cuh:317-322) -- the word becomes free.
+  // unlock_shared drops one holder (twopl.cuh:305-315), the last one leaves 0;
+  // unlock_exclusive frees the word (twopl.cuh:317-322).
   static __device__ void unlock(uint64_t* w, bool excl) {
     if (excl)
       st(w, 0);
     else
-      fetch_sub(w, 1ull << 1);
+      fetch_sub(w, tpl::CNT_ONE);
   }
 
   template <int G>
@@ -216,7 +224,7 @@ struct TwoPL {
       }
       commit = (c.ph == PH_COMMIT);
     }
-    uint64_t ticket = wave_fetch_inc(a.ctrs + 1, commit && (lane_id() & (G - 1)) == 0);
+    uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, commit && (lane_id() & (G - 1)) == 0);
     if (commit) {
       ticket = group_bcast64<G>(ticket);
       order_acquire();
@@ -279,7 +287,7 @@ struct TimestampOrdering {
     constexpr int K = TxCtx<G>::K;
     const uint64_t now = now_ticks();
     const bool start = !c.done && c.ph == PH_START && backoff_over(c, now);
-    uint64_t ts = wave_fetch_inc(a.ctrs + 0, start && (lane_id() & (G - 1)) == 0);
+    uint64_t ts = alloc_ts<G>(a, start);
     if (start) {
       c.ts = group_bcast64<G>(ts);
       c.m_ts += now_ticks() - now;
@@ -409,7 +417,7 @@ struct Mvcc {
     constexpr int K = TxCtx<G>::K;
     const uint64_t now = now_ticks();
     const bool start = !c.done && c.ph == PH_START && backoff_over(c, now);
-    uint64_t ts = wave_fetch_inc(a.ctrs + 0, start && (lane_id() & (G - 1)) == 0);
+    uint64_t ts = alloc_ts<G>(a, start);
     if (start) {
       c.ts = group_bcast64<G>(ts);
       c.m_ts += now_ticks() - now;
@@ -484,7 +492,7 @@ struct Mvcc {
 #pragma unroll
       for (int k = 0; k < K; ++k) nw += (c.meta[k] & M_HELD) ? 1u : 0u;
     }
-    uint64_t node = wave_fetch_add(a.ctrs + 2, nw);
+    uint64_t node = wave_fetch_add(a.ctrs + CTR_POOL, nw);
     if (!commit) return;
     if (node + nw > a.pool_cap) {
       atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_POOL_OVERFLOW);
@@ -570,27 +578,15 @@ struct Silo {
         for (int k = 0; k < K; ++k) {
           const uint32_t m = c.meta[k];
           if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_HELD)) continue;
-          uint64_t* w = a.cc + c.row[k];
-          uint64_t cur = ld(w);
-          StepResult r = R_WAIT;
-#pragma unroll 1
-          for (int t = 0; t < CAS_TRIES; ++t) {
-            if (silo::locked(cur)) {
-              r = R_ABORT;
-              break;
-            }
-            if (cas(w, cur, cur | 1ull)) {
-              r = R_OK;
-              break;
-            }
-          }
-          if (r == R_OK) {
+          // lock (silo.cuh:199-215) as one fetch_or: the old word says whether
+          // the lock was free (then it is ours) and carries the current TID;
+          // setting an already-set bit changes nothing, so no undo is needed
+          const uint64_t cur = fetch_or(a.cc + c.row[k], 1ull);
+          if (!silo::locked(cur)) {
             c.st[k] = silo::ts(cur);
             c.meta[k] = m | M_HELD;
-          } else if (r == R_ABORT) {
-            conflict = true;
           } else {
-            pending = true;
+            conflict = true;
           }
         }
         if (group_any<G>(conflict)) {
@@ -604,7 +600,7 @@ struct Silo {
       }
     }
     order_acquire();
-    uint64_t ticket = wave_fetch_inc(a.ctrs + 1, validate && (lane_id() & (G - 1)) == 0);
+    uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, validate && (lane_id() & (G - 1)) == 0);
     if (!validate) return;
     ticket = group_bcast64<G>(ticket);
     order_acquire();
@@ -694,27 +690,15 @@ struct Tictoc {
         for (int k = 0; k < K; ++k) {
           const uint32_t m = c.meta[k];
           if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_HELD)) continue;
-          uint64_t* w = a.cc + c.row[k];
-          uint64_t cur = ld(w);
-          StepResult r = R_WAIT;
-#pragma unroll 1
-          for (int t = 0; t < CAS_TRIES; ++t) {
-            if (tictoc::locked(cur)) {
-              r = R_ABORT;
-              break;
-            }
-            if (cas(w, cur, cur | 1ull)) {
-              r = R_OK;
-              break;
-            }
-          }
-          if (r == R_OK) {
+          // lock (tictoc.cuh:198-215) as one fetch_or.  A reader's rts
+          // extension never CASes a locked word (tictoc.cuh:316-338), so the
+          // old word stays valid until this attempt unlocks or writes.
+          const uint64_t cur = fetch_or(a.cc + c.row[k], 1ull);
+          if (!tictoc::locked(cur)) {
             c.st[k] = cur;
             c.meta[k] = m | M_HELD;
-          } else if (r == R_ABORT) {
-            conflict = true;
           } else {
-            pending = true;
+            conflict = true;
           }
         }
         if (group_any<G>(conflict)) {
@@ -769,7 +753,7 @@ struct Tictoc {
       }
     }
     order_acquire();
-    uint64_t ticket = wave_fetch_inc(a.ctrs + 1, commit && (lane_id() & (G - 1)) == 0);
+    uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, commit && (lane_id() & (G - 1)) == 0);
     if (!commit) return;
     ticket = group_bcast64<G>(ticket);
     // write phase (tictoc.cuh:352-368)

@@ -407,9 +407,10 @@ __global__ void k_reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_ta
 }
 __global__ void k_batch_begin(uint64_t* ctrs) {
   if (threadIdx.x == 0) {
-    dev::st(ctrs + 0, 1);  // timestamp allocator restarts at 1 (timestamp.cuh:76-80)
-    dev::st(ctrs + 1, 0);  // commit tickets
-    dev::st(ctrs + 2, 0);  // MVCC pool bump pointer
+    dev::st(ctrs + CTR_TICKET, 0);  // commit tickets
+    dev::st(ctrs + CTR_POOL, 0);    // MVCC pool bump pointer
+    // timestamp allocators restart at 1 (timestamp.cuh:76-80)
+    for (int s = 0; s < TS_SHARDS; ++s) dev::st(ctrs + CTR_TS0 + 16 * s, 1);
   }
 }
 

@@ -87,15 +87,19 @@ GC_HD uint64_t write_value(uint64_t seed, uint64_t tx, uint32_t op) {
 // ---------------------------------------------------------------------------
 // CC word layouts (explicit masks; the reference uses LSB-first bitfields).
 // ---------------------------------------------------------------------------
-// 2PL lock word (include/twopl.cuh:10-21): bit 0 shared, bits 1..31
-// holder_cnt, bits 32..62 holder.
+// 2PL lock word.  The reference packs {shared:1, holder_cnt:31, holder:31}
+// (include/twopl.cuh:10-21) and leaves stale bits in a free word.  Here bit 0
+// marks EXCLUSIVE, bits 1..31 count holders, bits 32..62 hold the exclusive
+// holder, and a free word is always 0: a shared lock adds one holder, the last
+// shared unlock subtracts back to 0, an exclusive unlock stores 0.  An
+// uncontended lock is then a single CAS that guesses the word is 0.
 namespace tpl {
-GC_HD uint64_t shared(uint64_t w) { return w & 1u; }
+constexpr uint64_t EXCL = 1ull;
+constexpr uint64_t CNT_ONE = 2ull;
+GC_HD uint64_t excl(uint64_t w) { return w & 1u; }
 GC_HD uint64_t cnt(uint64_t w) { return (w >> 1) & 0x7fffffffull; }
 GC_HD uint64_t holder(uint64_t w) { return (w >> 32) & 0x7fffffffull; }
-GC_HD uint64_t make(uint64_t sh, uint64_t c, uint64_t h) {
-  return (sh & 1u) | ((c & 0x7fffffffull) << 1) | ((h & 0x7fffffffull) << 32);
-}
+GC_HD uint64_t make_excl(uint64_t h) { return EXCL | CNT_ONE | ((h & 0x7fffffffull) << 32); }
 }  // namespace tpl
 
 // T/O and MVCC timestamp word (include/to.cuh:11-22, include/mvcc.cuh:12-23):
@@ -147,6 +151,17 @@ struct Metrics {
   uint64_t pad[6];
 };
 constexpr int METRICS_WORDS = 16;
+// Device counters (u64 indices into ExecArgs::ctrs, each on its own 128-byte
+// line).  T/O and MVCC only need unique timestamps, so the allocator is
+// sharded: shard s (by block index) hands out s, s+TS_SHARDS, s+2*TS_SHARDS,
+// ... which divides the traffic on any one address by TS_SHARDS.  Commit
+// tickets must follow a single real-time order and keep one counter.
+constexpr int CTR_TICKET = 0;
+constexpr int CTR_POOL = 16;
+constexpr int CTR_TS0 = 32;
+constexpr int TS_SHARDS = 16;
+constexpr int TS_SHARD_BITS = 4;
+constexpr int CTR_WORDS = CTR_TS0 + 16 * TS_SHARDS;
 // Waves add their totals into one of METRIC_SLOTS 128-byte slots (by block
 // index) so that the end-of-kernel atomics do not pile onto one address; the
 // host sums the slots.

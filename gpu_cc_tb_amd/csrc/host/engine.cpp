@@ -118,6 +118,7 @@ struct gcctb_ctx {
   gcctb::DevBuf<uint64_t> dbg;
   std::string dbg_path;
   uint64_t nranks = 0;
+  uint64_t last_committed = 0;
   std::vector<uint32_t> h_rank_counts;
   std::unique_ptr<gcctb::SchemeHost> host;
 };
@@ -396,8 +397,8 @@ void ensure_state(gcctb_ctx& c) {
     c.counters.alloc(rows);
     HIP_TRY(hipMemsetAsync(c.counters.p, 0, rows * 4, c.stream));
   }
-  c.ctrs.alloc(8);
-  HIP_TRY(hipMemsetAsync(c.ctrs.p, 0, 64, c.stream));
+  c.ctrs.alloc(CTR_WORDS);
+  HIP_TRY(hipMemsetAsync(c.ctrs.p, 0, CTR_WORDS * sizeof(uint64_t), c.stream));
   c.metrics.alloc(METRIC_SLOTS);
 }
 
@@ -764,6 +765,7 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
   st.error_flags = m.error_flags;
   st.total_ops = c->nops;
   st.read_ops = c->nreads;
+  c->last_committed = m.committed;
   if (out) *out = st;
   if (m.error_flags) {
     std::string what;
@@ -858,6 +860,30 @@ int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int
   (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0);
   if (wall_clock_khz) *wall_clock_khz = rate;
   if (cus) *cus = p.multiProcessorCount;
+  return GCCTB_OK;
+}
+
+int gcctb_check(gcctb_ctx* c, const uint64_t* init_table, uint64_t* bad, uint64_t* uncommitted) {
+  if (!c || !init_table || !bad) return set_error(GCCTB_EINVAL, "null argument");
+  const uint64_t ncells = c->cfg.rows * c->cfg.cols;
+  std::vector<uint64_t> fin(ncells), reads(c->nops + 1), order(3 * c->ntx + 1), ops(c->nops + 1);
+  std::vector<uint32_t> rows(c->nops + 1);
+  try {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(fin.data(), c->cells.p, ncells * 8, hipMemcpyDeviceToHost));
+    if (c->nops) {
+      HIP_TRY(hipMemcpy(reads.data(), c->reads.p, c->nops * 8, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(ops.data(), c->ops.p, c->nops * 8, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(rows.data(), c->rows_out.p, c->nops * 4, hipMemcpyDeviceToHost));
+    }
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  const int rc = gcctb_fetch(c, nullptr, nullptr, order.data(), nullptr);
+  if (rc != GCCTB_OK) return rc;
+  check_history(c->cfg.rows, c->cfg.cols, c->cfg.seed, init_table, fin.data(), ops.data(), rows.data(),
+                c->h_tx_off.data(), c->ntx, order.data(), reads.data(), bad);
+  if (uncommitted) *uncommitted = c->ntx - std::min(c->ntx, c->last_committed);
   return GCCTB_OK;
 }
 

@@ -15,3 +15,10 @@ int read_txs_file(const char* path, uint64_t ntx, uint32_t cols, uint64_t seed, 
 int write_txs_file(const char* path, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx);
 
 }  // namespace gcctb
+
+namespace gcctb {
+int check_history(uint64_t rows, uint32_t cols, uint64_t seed, const uint64_t* init_table,
+                  const uint64_t* final_table, const uint64_t* ops, const uint32_t* op_rows,
+                  const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, const uint64_t* reads,
+                  uint64_t* bad);
+}  // namespace gcctb

@@ -119,6 +119,13 @@ int gcctb_fetch_ranks(gcctb_ctx* ctx, uint32_t* rank, uint32_t* counts, uint64_t
 /* Per-row CC words after a run (2PL/T-O/MVCC/Silo/TicToc: one u64 per row). */
 int gcctb_fetch_cc(gcctb_ctx* ctx, uint64_t* words);
 
+/* Serializability check of the last run (the driver's debug mode;
+ * ExecInfo::Check, src/runtime.cu:129-158): re-executes the committed
+ * transactions one by one in the reported order from init_table (the table as
+ * it was before the run) and counts reads and final cells that disagree with
+ * the GPU (*bad) and transactions that did not commit (*uncommitted). */
+int gcctb_check(gcctb_ctx* ctx, const uint64_t* init_table, uint64_t* bad, uint64_t* uncommitted);
+
 /* Workload generation (host, deterministic in (seed, tx, draw); counter-based,
  * multi-threaded).  Zipf draw follows generation/gen_ycsb.cu:76-102 with
  * n = rows and row = rank - 1; keys are distinct inside a transaction and ops
