@@ -54,9 +54,10 @@ def parse():
     return p.parse_args()
 
 
-# launch shape per scheme (warp_density, block_size); chosen by the sweep in
-# the autotuner for the low-contention workload
-DEFAULT_SHAPE = {s: (4, 8) for s in g.SCHEMES}
+# launch shape per scheme (warp_density, block_size): best of the wd x bs sweep
+# on this workload (wd 1..4 x bs 4, 8) on MI355X
+DEFAULT_SHAPE = {"tpl_nw": (3, 8), "tpl_wd": (3, 8), "to": (4, 8), "mvcc": (4, 4), "silo": (3, 8),
+                 "tictoc": (3, 8), "gacco": (2, 8), "gputx": (3, 8)}
 
 
 def algorithmic_bytes(ntx: int, nops: int, nreads: int) -> int:

@@ -38,7 +38,7 @@ class Config(C.Structure):
         ("cols", C.c_uint32),
         ("seed", C.c_uint64),
         ("backoff_ticks", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("persist_window", C.c_uint32),
     ]
 
 
@@ -91,6 +91,9 @@ def load_library() -> C.CDLL:
         "gcctb_probe": (C.c_int, [P, _u32p, C.c_uint64, _u32p]),
         "gcctb_load_ops": (C.c_int, [P, _u64p, _u64p, C.c_uint64]),
         "gcctb_run_plain": (C.c_int, [P, C.POINTER(Stats)]),
+        "gcctb_run_graph": (C.c_int, [P, C.POINTER(Stats)]),
+        "gcctb_run_persistent": (C.c_int, [P, C.POINTER(Stats)]),
+        "gcctb_set_batches": (C.c_int, [P, _u64p, C.c_uint64]),
         "gcctb_fetch": (C.c_int, [P, P, P, P, P]),
         "gcctb_fetch_ranks": (C.c_int, [P, P, P, C.POINTER(C.c_uint64)]),
         "gcctb_fetch_cc": (C.c_int, [P, _u64p]),
@@ -219,12 +222,12 @@ class Context:
 
     def __init__(self, cc_scheme: str, *, rows: int, cols: int = 10, warp_density: int = 5,
                  block_size: int = 32, batch_size: int = 65536, seed: int = SEED,
-                 backoff_ticks: int = 20):
+                 backoff_ticks: int = 20, persist_window: int = 0):
         self._lib = load_library()
         self.scheme = cc_scheme
         self.rows, self.cols, self.seed = rows, cols, seed
         cfg = Config(cc_scheme.encode(), warp_density, block_size, batch_size, rows, cols, seed,
-                     backoff_ticks, 0)
+                     backoff_ticks, persist_window)
         h = C.c_void_p()
         _check(self._lib.gcctb_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -273,14 +276,25 @@ class Context:
         _check(self._lib.gcctb_load_ops(self._h, ops, np.ascontiguousarray(w.tx_off, dtype=np.uint64), w.ntx))
         self.ntx, self.nops = w.ntx, w.nops
 
-    def run_plain(self, check: bool = True) -> dict:
+    def run(self, form: str = "plain", check: bool = True) -> dict:
+        """form: 'plain' (per-batch launches), 'graph' (captured batch sequence)
+        or 'persistent' (one launch over the whole stream)."""
         st = Stats()
-        rc = self._lib.gcctb_run_plain(self._h, C.byref(st))
+        fn = {"plain": self._lib.gcctb_run_plain, "graph": self._lib.gcctb_run_graph,
+              "persistent": self._lib.gcctb_run_persistent}[form]
+        rc = fn(self._h, C.byref(st))
         if check:
             _check(rc)
         d = st.as_dict()
         d["rc"] = rc
         return d
+
+    def run_plain(self, check: bool = True) -> dict:
+        return self.run("plain", check)
+
+    def set_batches(self, sizes) -> None:
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+        _check(self._lib.gcctb_set_batches(self._h, sizes if sizes.size else np.zeros(1, np.uint64), sizes.size))
 
     def fetch(self, table: bool = True, reads: bool = True, order: bool = True, aborts: bool = False):
         t = np.zeros(self.rows * self.cols, dtype=np.uint64) if table else None

@@ -55,7 +55,7 @@ struct TxCtx {
   bool done = true;
   uint64_t ts = 0;         // attempt timestamp (T/O, MVCC) or commit timestamp
   uint64_t t_start = 0, t_attempt = 0, t_resume = 0, t_round = 0;
-  uint64_t m_ts = 0, m_wait = 0, m_abort = 0, m_idx = 0, m_tot = 0, m_commit = 0;
+  uint64_t m_ts = 0, m_wait = 0, m_abort = 0, m_idx = 0, m_tot = 0, m_commit = 0, m_att = 0, m_miss = 0;
   uint32_t row[K];
   uint32_t meta[K];
   uint64_t st[K];
@@ -99,6 +99,7 @@ __device__ __forceinline__ void finish_tx(TxCtx<G>& c, const ExecArgs& a, uint64
   c.done = true;
   c.m_tot += now - c.t_start;
   c.m_commit += 1;
+  c.m_att += c.attempts;
   if ((lane_id() & (G - 1)) == 0) {
     a.obatch[c.tx] = a.batch_id;
     a.okey0[c.tx] = key0;
@@ -129,6 +130,18 @@ __device__ __forceinline__ uint64_t alloc_ts(const ExecArgs& a, bool start) {
   const uint32_t shard = blockIdx.x & (TS_SHARDS - 1);
   const uint64_t local = wave_fetch_inc(a.ctrs + CTR_TS0 + 16 * shard, start && (lane_id() & (G - 1)) == 0);
   return (local << TS_SHARD_BITS) | shard;
+}
+
+// After an abort caused by a younger timestamp `seen`, move this block's
+// shard past it, so that the retry's timestamp is younger than what it lost
+// to.  Without it a slow shard's transactions could keep losing to the
+// timestamps of a faster shard (a single counter never lags itself).
+template <int G>
+__device__ __forceinline__ void raise_ts_floor(const ExecArgs& a, uint64_t seen) {
+  if (seen == 0 || (lane_id() & (G - 1)) != 0) return;
+  const uint32_t shard = blockIdx.x & (TS_SHARDS - 1);
+  __hip_atomic_fetch_max(a.ctrs + CTR_TS0 + 16 * shard, (seen >> TS_SHARD_BITS) + 1, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ===========================================================================
@@ -257,11 +270,14 @@ struct TwoPL {
 // serialization order is timestamp order.
 // ===========================================================================
 struct TimestampOrdering {
-  static __device__ StepResult read(uint64_t* w, uint64_t* cell, uint64_t ts, uint64_t& val) {
+  static __device__ StepResult read(uint64_t* w, uint64_t* cell, uint64_t ts, uint64_t& val, uint64_t& seen) {
     uint64_t cur = ld(w);
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
-      if (tsw::wts(cur) > ts) return R_ABORT;
+      if (tsw::wts(cur) > ts) {
+        seen = tsw::wts(cur);
+        return R_ABORT;
+      }
       if (tsw::unc(cur)) return R_WAIT;
       order_acquire();
       val = ld(cell);
@@ -271,17 +287,22 @@ struct TimestampOrdering {
     }
     return R_WAIT;
   }
-  static __device__ StepResult write_mark(uint64_t* w, uint64_t ts, uint64_t& prev) {
+  static __device__ StepResult write_mark(uint64_t* w, uint64_t ts, uint64_t& prev, uint64_t& seen) {
     uint64_t cur = ld(w);
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
-      if (tsw::rts(cur) > ts || tsw::wts(cur) > ts || tsw::unc(cur)) return R_ABORT;
+      if (tsw::rts(cur) > ts || tsw::wts(cur) > ts || tsw::unc(cur)) {
+        seen = tsw::rts(cur) > tsw::wts(cur) ? tsw::rts(cur) : tsw::wts(cur);
+        return R_ABORT;
+      }
       prev = cur;
       if (cas(w, cur, tsw::make(1, ts, ts))) return R_OK;
     }
     return R_WAIT;
   }
 
+  // Reads first, write marks second: an attempt never waits while it holds
+  // an uncommitted mark, so no chain of waits forms through it.
   template <int G>
   static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
     constexpr int K = TxCtx<G>::K;
@@ -299,24 +320,17 @@ struct TimestampOrdering {
       begin_attempt(c, now);
       c.ph = PH_OPS;
     }
-    if (c.done || c.ph != PH_OPS) return;
-    bool abort = false, pending = false;
+    if (c.done) return;
+    uint64_t seen = 0;
+    bool abort = false;
+    if (c.ph == PH_OPS) {
+      bool pending = false;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t m = c.meta[k];
-      if (!(m & M_VALID) || (m & M_DONE)) continue;
-      uint64_t* w = a.cc + c.row[k];
-      if (m & M_WRITE) {
-        const StepResult r = write_mark(w, c.ts, c.st[k]);
-        if (r == R_OK)
-          c.meta[k] = m | M_DONE | M_HELD;
-        else if (r == R_ABORT)
-          abort = true;
-        else
-          pending = true;
-      } else {
+      for (int k = 0; k < K; ++k) {
+        const uint32_t m = c.meta[k];
+        if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
         uint64_t v;
-        const StepResult r = read(w, cell_ptr(a, c.row[k], m), c.ts, v);
+        const StepResult r = read(a.cc + c.row[k], cell_ptr(a, c.row[k], m), c.ts, v, seen);
         if (r == R_OK) {
           c.meta[k] = m | M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v;
@@ -326,19 +340,39 @@ struct TimestampOrdering {
           pending = true;
         }
       }
+      if (!group_any<G>(abort)) {
+        if (group_all<G>(!pending))
+          c.ph = PH_WRITES;
+        else
+          c.m_wait += now - c.t_round;
+      }
+    }
+    if (!abort && c.ph == PH_WRITES) {
+      bool pending = false;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint32_t m = c.meta[k];
+        if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_DONE)) continue;
+        const StepResult r = write_mark(a.cc + c.row[k], c.ts, c.st[k], seen);
+        if (r == R_OK)
+          c.meta[k] = m | M_DONE | M_HELD;
+        else if (r == R_ABORT)
+          abort = true;
+        else
+          pending = true;
+      }
+      if (!group_any<G>(abort) && !group_all<G>(!pending)) return;
     }
     if (group_any<G>(abort)) {
       // rollback (to.cuh:255-270): restore the words this attempt marked
 #pragma unroll
       for (int k = 0; k < K; ++k)
         if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+      raise_ts_floor<G>(a, group_max64<G>(seen));
       abort_attempt(c, a, now);
       return;
     }
-    if (!group_all<G>(!pending)) {
-      c.m_wait += now - c.t_round;
-      return;
-    }
+    if (c.ph != PH_WRITES) return;
     // commit (to.cuh:117-134)
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -451,6 +485,7 @@ struct Mvcc {
     }
     if (!c.done && c.ph == PH_WRITES) {
       bool abort = false, pending = false;
+      uint64_t seen = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const uint32_t m = c.meta[k];
@@ -461,6 +496,8 @@ struct Mvcc {
 #pragma unroll 1
         for (int t = 0; t < CAS_TRIES; ++t) {
           if (tsw::rts(cur) > c.ts || tsw::wts(cur) > c.ts || tsw::unc(cur)) {
+            const uint64_t y = tsw::rts(cur) > tsw::wts(cur) ? tsw::rts(cur) : tsw::wts(cur);
+            seen = y > seen ? y : seen;
             r = R_ABORT;
             break;
           }
@@ -481,6 +518,7 @@ struct Mvcc {
 #pragma unroll
         for (int k = 0; k < K; ++k)
           if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+        raise_ts_floor<G>(a, group_max64<G>(seen));
         abort_attempt(c, a, now);
       } else if (group_all<G>(!pending)) {
         commit = true;
@@ -823,23 +861,31 @@ struct Ordered {
 
 // ===========================================================================
 // The kernel.
+//
+// Plain form (PERSIST = false): one launch per batch, one transaction per
+// group, the reference's launch structure (benchmark_cpu/ycsb.cu:262-285).
+// Persistent form (PERSIST = true): one launch over the whole stream with as
+// many groups as the device holds; group g runs slots g, g + NG, g + 2NG, ...
+// and loads its next transaction as soon as the current one commits, so no
+// group idles behind a batch's slowest transaction.  Each group takes its slots
+// in order, so the lowest unfinished slot is always resident, which the
+// ordered schemes (waits on lower slots only) need for progress.
 // ===========================================================================
-template <class S, int G>
-__global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
+template <int G>
+__device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t slot, uint64_t now) {
   constexpr int K = TxCtx<G>::K;
-  TxCtx<G> c;
-  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t slot = a.slot_lo + gtid / G;
   c.live = slot < a.slot_hi;
   c.done = !c.live;
+  c.ph = PH_START;
+  c.attempts = 0;
+  c.t_resume = 0;
+  c.t_start = now;
+  c.nops = 0;
   if (c.live) {
     c.tx = a.tx_map ? a.tx_map[slot] : slot;
     c.op0 = a.tx_off[c.tx];
     c.nops = (uint32_t)(a.tx_off[c.tx + 1] - c.op0);
   }
-  c.t_start = now_ticks();
-  c.t_round = c.t_start;
-  uint64_t misses = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint32_t j = op_index<G>(k);
@@ -854,22 +900,43 @@ __global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
         c.row[k] = row;
         c.meta[k] = M_VALID | (op_is_write(op) ? M_WRITE : 0u) | (op_col(op) << M_COL_SHIFT);
       } else {
-        misses++;
+        c.m_miss++;
       }
     }
   }
-  if (c.live && c.nops > (uint32_t)(K * G)) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_TXLEN);
-  c.m_idx = now_ticks() - c.t_start;
+  if (c.live) {
+    if (c.nops > (uint32_t)(K * G)) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_TXLEN);
+    c.m_idx += now_ticks() - now;
+  }
+}
+
+template <class S, int G, bool PERSIST>
+__global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
+  constexpr int K = TxCtx<G>::K;
+  TxCtx<G> c;
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
+  uint64_t slot = a.slot_lo + gtid / G;
+  c.t_round = now_ticks();
+  load_tx(c, a, slot, c.t_round);
 #pragma unroll 1
-  while (ballot(!c.done) != 0) {
+  while (true) {
+    if (PERSIST && c.done && slot < a.slot_hi) {
+      slot += ngroups;
+      if (slot < a.slot_hi) load_tx(c, a, slot, now_ticks());
+    }
+    if (ballot(!c.done) == 0) break;
     // every live group of the wave is backing off: sleep instead of spinning
     if (ballot(!c.done && !(c.ph == PH_START && c.t_round < c.t_resume)) == 0)
       __builtin_amdgcn_s_sleep(4);
     else
       S::round(c, a);
     c.t_round = now_ticks();
-    // Watchdog: the wall clock is a scalar read, so the exit is wave-uniform.
-    if (a.watchdog_ticks && c.t_round - c.t_start > a.watchdog_ticks) {
+    // Watchdog on the oldest live transaction of the wave (t_start is reset
+    // when a group loads a transaction); the wall clock is a scalar read, so
+    // the exit is wave-uniform.
+    const bool stuck = !c.done && a.watchdog_ticks && c.t_round - c.t_start > a.watchdog_ticks;
+    if (ballot(stuck) != 0) {
       if (lane_id() == 0) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_WATCHDOG);
       if (a.dbg && !c.done) {
         // per op of a stopped transaction: flags/row, the CC word through the
@@ -888,13 +955,12 @@ __global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
       break;
     }
   }
-  // metrics: one value per transaction (group leader), reduced over the wave
-  const bool leader = c.live && (lane_id() & (G - 1)) == 0;
-  uint64_t v[8] = {leader ? c.attempts : 0ull, leader ? c.m_ts : 0ull, leader ? c.m_wait : 0ull,
+  // metrics: sums over the transactions the group ran, reduced over the wave
+  const bool leader = (lane_id() & (G - 1)) == 0;
+  uint64_t v[8] = {leader ? c.m_att : 0ull, leader ? c.m_ts : 0ull, leader ? c.m_wait : 0ull,
                    leader ? c.m_abort : 0ull, leader ? (c.m_tot - c.m_idx) : 0ull,
                    leader ? c.m_tot : 0ull,   leader ? c.m_idx : 0ull, leader ? c.m_commit : 0ull};
-  misses = group_sum64<G>(misses);
-  const uint64_t miss_leader = leader ? misses : 0ull;
+  const uint64_t misses = c.m_miss;  // per lane (each lane probes its own ops)
   // A block of block_size*32 threads can end in a half wave (lanes 0..31);
   // the last butterfly step then has no partner lanes and is skipped.
   const bool full_wave = (ballot(true) >> 32) != 0;
@@ -906,7 +972,7 @@ __global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
     if (full_wave) x += shfl_xor64(x, 32);
     v[f] = x;
   }
-  uint64_t mm = miss_leader;
+  uint64_t mm = misses;
 #pragma unroll
   for (int m = 1; m < 32; m <<= 1) mm += shfl_xor64(mm, m);
   if (full_wave) mm += shfl_xor64(mm, 32);

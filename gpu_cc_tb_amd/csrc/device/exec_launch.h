@@ -1,5 +1,6 @@
 // Launch helper shared by the per-scheme translation units: one kernel
-// instantiation per lanes-per-transaction G in {1, 2, 4, 8, 16, 32}.
+// instantiation per lanes-per-transaction G in {1, 2, 4, 8, 16, 32} and per
+// form (plain per-batch / persistent).
 #pragma once
 #include "exec_core.h"
 #include "launch.h"
@@ -7,24 +8,51 @@
 namespace gcctb {
 namespace dev {
 
-template <class S>
-hipError_t launch_exec_g(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s) {
-  if (nslots == 0) return hipSuccess;
+template <class S, int G, bool P>
+hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s) {
   const uint64_t total = nslots * (uint64_t)G;
-  const uint64_t blocks = (total + threads - 1) / threads;
-  if (blocks == 0 || blocks > 0x7fffffffull || threads <= 0 || threads > 1024)
-    return hipErrorInvalidValue;
-  const dim3 grid((unsigned)blocks), block((unsigned)threads);
+  uint64_t blocks = (total + threads - 1) / threads;
+  if (P) {
+    // a persistent grid holds no more blocks than the device keeps resident,
+    // so that every group is running and the slot order guarantees progress
+    int per_cu = 0, cus = 0, dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&txn_exec<S, G, P>),
+                                                       threads, 0);
+    if (e != hipSuccess) return e;
+    const uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 1) * (uint64_t)(cus > 0 ? cus : 1);
+    if (blocks > resident) blocks = resident;
+    // admission window: fewer transactions in flight under heavy contention
+    if (a.persist_groups) {
+      const uint64_t wblocks = (a.persist_groups * G + threads - 1) / threads;
+      if (wblocks < blocks) blocks = wblocks < 1 ? 1 : wblocks;
+    }
+  }
+  if (blocks == 0 || blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((txn_exec<S, G, P>), dim3((unsigned)blocks), dim3((unsigned)threads), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_exec_g(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s) {
+  if (nslots == 0) return hipSuccess;
+  if (threads <= 0 || threads > 1024) return hipErrorInvalidValue;
+#define GC_CASE(g)                                                                  \
+  case g:                                                                           \
+    return persist ? launch_one<S, g, true>(a, nslots, threads, s)                  \
+                   : launch_one<S, g, false>(a, nslots, threads, s);
   switch (G) {
-    case 1: hipLaunchKernelGGL((txn_exec<S, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((txn_exec<S, 2>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((txn_exec<S, 4>), grid, block, 0, s, a); break;
-    case 8: hipLaunchKernelGGL((txn_exec<S, 8>), grid, block, 0, s, a); break;
-    case 16: hipLaunchKernelGGL((txn_exec<S, 16>), grid, block, 0, s, a); break;
-    case 32: hipLaunchKernelGGL((txn_exec<S, 32>), grid, block, 0, s, a); break;
+    GC_CASE(1)
+    GC_CASE(2)
+    GC_CASE(4)
+    GC_CASE(8)
+    GC_CASE(16)
+    GC_CASE(32)
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
+#undef GC_CASE
 }
 
 }  // namespace dev

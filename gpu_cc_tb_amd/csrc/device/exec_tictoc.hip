@@ -2,8 +2,8 @@
 #include "exec_launch.h"
 namespace gcctb {
 namespace launch {
-hipError_t exec_tictoc(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s) {
-  return dev::launch_exec_g<dev::Tictoc>(G, a, nslots, threads, s);
+hipError_t exec_tictoc(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s) {
+  return dev::launch_exec_g<dev::Tictoc>(G, a, nslots, threads, persist, s);
 }
 }  // namespace launch
 }  // namespace gcctb

@@ -10,12 +10,12 @@ namespace launch {
 
 // --- execution kernels (one translation unit per scheme family) ---
 // G must be 1, 2, 4, 8, 16 or 32; threads = block threads.
-hipError_t exec_tpl(bool wait_die, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
-hipError_t exec_to(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
-hipError_t exec_mvcc(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
-hipError_t exec_silo(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
-hipError_t exec_tictoc(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
-hipError_t exec_ordered(int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_tpl(bool wait_die, int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
+hipError_t exec_to(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
+hipError_t exec_mvcc(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
+hipError_t exec_silo(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
+hipError_t exec_tictoc(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
+hipError_t exec_ordered(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
 
 // --- table / index ---
 hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s);
@@ -75,7 +75,7 @@ hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, 
 hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                          uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s);
 hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s);
-hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t* hist, hipStream_t s);
+hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks, uint32_t* hist, hipStream_t s);
 
 // --- memory-system ceilings (microbench.hip) ---
 hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint64_t iters, double* rate,

@@ -25,6 +25,21 @@ inline unsigned grid_stride_blocks(uint64_t n) {
   return (unsigned)g;
 }
 
+// Wave reductions (the streaming kernels run full 256-thread blocks).
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += dev::shfl_xor64(v, m);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, m, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 __global__ void k_seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed) {
   const uint64_t n = rows * cols;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -353,7 +368,11 @@ __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const u
         local++;
     }
   }
-  if (conflicts && local) atomicAdd(conflicts, (unsigned long long)local);
+  // one atomic per wave: a single counter sustains only ~1e8 atomics/s
+  if (conflicts) {
+    local = wave_sum_u64(local);
+    if (dev::lane_id() == 0 && local) atomicAdd(conflicts, (unsigned long long)local);
+  }
 }
 
 __global__ void k_gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
@@ -373,10 +392,11 @@ __global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t local = S[p] - S[SS[p]];
-    atomicMax(&rank[op_tx[sop[p]]], local);
+    if (local) atomicMax(&rank[op_tx[sop[p]]], local);  // ranks start at 0
     mx = local > mx ? local : mx;
   }
-  atomicMax(max_rank, mx);
+  mx = wave_max_u32(mx);
+  if (dev::lane_id() == 0 && mx) atomicMax(max_rank, mx);
 }
 __global__ void k_gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                                 uint64_t n, int rank_bits, uint64_t* keys) {
@@ -389,10 +409,28 @@ __global__ void k_u64_high(const uint64_t* keys, uint64_t n, int shift, uint32_t
        i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = (uint32_t)(keys[i] >> shift);
 }
-__global__ void k_rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t* hist) {
+// Histogram of ranks (gputx_pre3, src/gputx.cu:30-38).  There are usually few
+// ranks, so each block counts into LDS and adds its non-zero bins once.
+constexpr uint32_t RANK_LDS_BINS = 8192;
+__global__ __launch_bounds__(256) void k_rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks,
+                                                   uint32_t* hist) {
+  __shared__ uint32_t h[RANK_LDS_BINS];
+  const bool in_lds = nranks <= RANK_LDS_BINS;
+  if (in_lds)
+    for (uint32_t b = threadIdx.x; b < nranks; b += blockDim.x) h[b] = 0;
+  __syncthreads();
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntx;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&hist[rank[i]], 1u);
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rank[i];
+    if (in_lds)
+      atomicAdd(&h[r], 1u);
+    else
+      atomicAdd(&hist[r], 1u);
+  }
+  __syncthreads();
+  if (in_lds)
+    for (uint32_t b = threadIdx.x; b < nranks; b += blockDim.x)
+      if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
 __global__ void k_reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
@@ -502,8 +540,12 @@ hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint
 hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s) {
   GC_LAUNCH(k_u64_high, n, keys, n, shift, out);
 }
-hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t* hist, hipStream_t s) {
-  GC_LAUNCH(k_rank_hist, ntx, rank, ntx, hist);
+hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks, uint32_t* hist, hipStream_t s) {
+  uint64_t blocks = (ntx + 4095) / 4096;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_rank_hist, dim3((unsigned)blocks), dim3(256), 0, s, rank, ntx, nranks, hist);
+  return hipGetLastError();
 }
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
                       uint32_t* vprev, hipStream_t s) {

@@ -228,6 +228,7 @@ struct ExecArgs {
   const uint32_t* tx_rank; // gputx: per-tx rank (okey0)
   uint64_t watchdog_ticks; // a wave gives up after this many ticks (0 = never)
   uint64_t* dbg;           // optional: 4 u64 per op, written for transactions the watchdog stopped
+  uint64_t persist_groups; // persistent form: at most this many transactions in flight (0 = device-filling)
 };
 
 }  // namespace gcctb

@@ -120,6 +120,10 @@ struct gcctb_ctx {
   uint64_t nranks = 0;
   uint64_t last_committed = 0;
   std::vector<uint32_t> h_rank_counts;
+  std::vector<uint64_t> custom_batches;  // uneven batch sizes (empty: batch_size)
+  // hipGraph of the per-batch sequence (graph form), rebuilt when inputs change
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_gen = 0, inputs_gen = 1;
   std::unique_ptr<gcctb::SchemeHost> host;
 };
 
@@ -138,15 +142,24 @@ class SchemeHost {
   virtual ~SchemeHost() = default;
   virtual void Init0() {
     batches.clear();
+    if (!ctx.custom_batches.empty()) {  // uneven batches of a stream (gcctb_set_batches)
+      batches = ctx.custom_batches;
+      return;
+    }
     const uint64_t bs = ctx.cfg.batch_size;
     for (uint64_t st = 0; st < ctx.ntx; st += bs) batches.push_back(std::min(bs, ctx.ntx - st));
   }
+  // Persistent form: prepare the whole stream as one unit (no per-batch
+  // resets: timestamps and tickets keep counting across batch boundaries).
+  virtual void PrepStream() { HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream)); }
+  // Persistent form: restore the words every transaction of the stream touched.
+  virtual void PostStream() { Post(0, 0, ctx.ntx); }
   virtual void Init(int batch_id, uint64_t batch_st) {
     (void)batch_id;
     (void)batch_st;
     HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream));
   }
-  virtual void Launch(int batch_id, uint64_t slot_lo, uint64_t slot_hi) = 0;
+  virtual void Launch(int batch_id, uint64_t slot_lo, uint64_t slot_hi, bool persist = false) = 0;
   virtual void Post(int batch_id, uint64_t batch_st, uint64_t n) {
     (void)batch_id;
     const uint64_t o0 = ctx.h_tx_off[batch_st], o1 = ctx.h_tx_off[batch_st + n];
@@ -190,6 +203,7 @@ class SchemeHost {
     a.tx_rank = nullptr;
     a.watchdog_ticks = ctx.watchdog_ticks;
     a.dbg = ctx.dbg.p;
+    a.persist_groups = ctx.cfg.persist_window;
     return a;
   }
   gcctb_ctx& ctx;
@@ -198,8 +212,8 @@ class SchemeHost {
 class TwoPLHost : public SchemeHost {
  public:
   TwoPLHost(gcctb_ctx& c, bool wd) : SchemeHost(c), wait_die(wd) {}
-  void Launch(int b, uint64_t lo, uint64_t hi) override {
-    HIP_TRY(launch::exec_tpl(wait_die, ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
+    HIP_TRY(launch::exec_tpl(wait_die, ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
   // Locks are free again once every transaction of the batch committed.
   void Post(int, uint64_t, uint64_t) override {}
@@ -208,30 +222,30 @@ class TwoPLHost : public SchemeHost {
 class TOHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
-  void Launch(int b, uint64_t lo, uint64_t hi) override {
-    HIP_TRY(launch::exec_to(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
+    HIP_TRY(launch::exec_to(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
 };
 class MVCCHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
-  void Launch(int b, uint64_t lo, uint64_t hi) override {
-    HIP_TRY(launch::exec_mvcc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
+    HIP_TRY(launch::exec_mvcc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
   uint64_t GetMemSize() const override { return ctx.cc.n * 8 + ctx.vprev.n * 4 + ctx.pool.n * 8; }
 };
 class SiloHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
-  void Launch(int b, uint64_t lo, uint64_t hi) override {
-    HIP_TRY(launch::exec_silo(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
+    HIP_TRY(launch::exec_silo(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
 };
 class TictocHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
-  void Launch(int b, uint64_t lo, uint64_t hi) override {
-    HIP_TRY(launch::exec_tictoc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
+    HIP_TRY(launch::exec_tictoc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
 };
 
@@ -242,9 +256,16 @@ class GaccoHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
   void Init(int batch_id, uint64_t batch_st) override {
-    SchemeHost::Init(batch_id, batch_st);
     const uint64_t n_tx = batches[batch_id];
-    const uint64_t o0 = ctx.h_tx_off[batch_st], o1 = ctx.h_tx_off[batch_st + n_tx];
+    prep_range(ctx.h_tx_off[batch_st], ctx.h_tx_off[batch_st + n_tx]);
+  }
+  // The persistent form queues every row's accesses of the whole stream in
+  // transaction-id order: the outcome is that of the serial run in id order,
+  // the same as batch after batch.
+  void PrepStream() override { prep_range(0, ctx.nops); }
+  void PostStream() override {}
+  // per-row access queues for ops [o0, o1) in (row, transaction id) order
+  void prep_range(uint64_t o0, uint64_t o1) {
     const uint64_t n = o1 - o0;
     if (n == 0) return;
     hipStream_t s = ctx.stream;
@@ -258,8 +279,8 @@ class GaccoHost : public SchemeHost {
     HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows,
                                 nullptr, nullptr, ctx.ops.p, nullptr, s));
   }
-  void Launch(int b, uint64_t lo, uint64_t hi) override {
-    HIP_TRY(launch::exec_ordered(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, ctx.stream));
+  void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
+    HIP_TRY(launch::exec_ordered(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
   void Post(int, uint64_t, uint64_t) override {}
   uint64_t GetMemSize() const override { return ctx.counters.n * 4 + ctx.need.n * 4; }
@@ -300,7 +321,7 @@ class GPUTxHost : public SchemeHost {
       DevBuf<uint32_t> hist;
       hist.alloc(ctx.nranks);
       HIP_TRY(launch::fill_u32(hist.p, ctx.nranks, 0, s));
-      HIP_TRY(launch::rank_hist(ctx.rank.p, ntx, hist.p, s));
+      HIP_TRY(launch::rank_hist(ctx.rank.p, ntx, (uint32_t)ctx.nranks, hist.p, s));
       HIP_TRY(hipMemcpyAsync(ctx.h_rank_counts.data(), hist.p, ctx.nranks * 4, hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
     } else if (ntx) {
@@ -334,11 +355,14 @@ class GPUTxHost : public SchemeHost {
     }
   }
   void Init(int, uint64_t) override {}
-  void Launch(int, uint64_t lo, uint64_t hi) override {
+  // the persistent form runs every rank level in one launch
+  void PrepStream() override { Init0(); }
+  void PostStream() override {}
+  void Launch(int, uint64_t lo, uint64_t hi, bool persist) override {
     ExecArgs a = args(0, lo, hi);
     a.tx_map = ctx.tx_map.p;
     a.tx_rank = ctx.rank.p;
-    HIP_TRY(launch::exec_ordered(ctx.G, a, hi - lo, ctx.threads, ctx.stream));
+    HIP_TRY(launch::exec_ordered(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
   void Post(int, uint64_t, uint64_t) override {}
   uint64_t GetMemSize() const override { return ctx.counters.n * 4 + ctx.need.n * 4 + ctx.rank.n * 8; }
@@ -470,6 +494,7 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
 void gcctb_destroy(gcctb_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   c->host.reset();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -588,18 +613,15 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
   if (nops >= (1ull << 32)) return set_error(GCCTB_EINVAL, "at most 2^32-1 ops");
   // MVCC version pool: the largest number of writes in one batch
   uint64_t cap = 0;
-  if (c->scheme == SCHEME_MVCC) {
-    for (uint64_t st = 0; st < ntx; st += c->cfg.batch_size) {
-      const uint64_t en = std::min(ntx, st + c->cfg.batch_size);
-      uint64_t w = 0;
-      for (uint64_t i = tx_off[st]; i < tx_off[en]; ++i) w += op_is_write(ops[i]) ? 1 : 0;
-      cap = std::max(cap, w);
-    }
-  }
+  // MVCC version pool: one node per write of the whole load (the persistent
+  // form keeps every version of the stream; a plain batch needs at most that)
+  if (c->scheme == SCHEME_MVCC) cap = nops - nreads;
   try {
     c->ntx = ntx;
     c->nops = nops;
     c->nreads = nreads;
+    c->inputs_gen++;
+    c->custom_batches.clear();
     c->h_tx_off.assign(tx_off, tx_off + ntx + 1);
     c->ops.alloc(nops);
     c->tx_off.alloc(ntx + 1);
@@ -624,16 +646,8 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
       c->pool.alloc(3 * cap + 3);
     }
     if (c->scheme == SCHEME_GACCO || c->scheme == SCHEME_GPUTX) {
-      const uint64_t n = c->scheme == SCHEME_GACCO
-                             ? [&] {
-                                 uint64_t m = 0;
-                                 for (uint64_t st = 0; st < ntx; st += c->cfg.batch_size) {
-                                   const uint64_t en = std::min(ntx, st + c->cfg.batch_size);
-                                   m = std::max(m, tx_off[en] - tx_off[st]);
-                                 }
-                                 return m;
-                               }()
-                             : std::max(nops, ntx);
+      // sort buffers for the whole load (the persistent form prepares it at once)
+      const uint64_t n = std::max(nops, ntx);
       c->need.alloc(nops);
       c->sk0.alloc(n);
       c->sv0.alloc(n);
@@ -660,7 +674,15 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
   return GCCTB_OK;
 }
 
-int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
+enum RunForm { FORM_PLAIN = 0, FORM_GRAPH = 1, FORM_PERSIST = 2 };
+
+// One pass over every loaded transaction.
+//   plain:      the reference's structure -- per batch Init, one launch, Post,
+//               host synchronization (benchmark_cpu/ycsb.cu:262-285)
+//   graph:      the same per-batch kernels captured once into a hipGraph and
+//               replayed, with no host round trip between batches
+//   persistent: one preparation of the whole stream and one persistent launch
+static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
   if (!c) return set_error(GCCTB_EINVAL, "null context");
   if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
   gcctb_stats st{};
@@ -671,14 +693,65 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0;
     double pre = 0, proc = 0;
+    uint64_t launches = 0;
     HIP_TRY(hipEventRecord(c->ev[0], s));
+    if (form == FORM_PERSIST) {
+      c->host->PrepStream();
+      HIP_TRY(hipEventRecord(c->ev[3], s));
+      c->host->Launch(0, 0, c->ntx, true);
+      HIP_TRY(hipEventRecord(c->ev[4], s));
+      c->host->PostStream();
+      HIP_TRY(hipEventRecord(c->ev[5], s));
+      HIP_TRY(hipEventSynchronize(c->ev[5]));
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
+      pre += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+      pre += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+      proc += ms;
+      launches = 1;
+    } else if (form == FORM_GRAPH) {
+      // Init0 may synchronize with the host (GPUTx ranks): it stays outside
+      c->host->Init0();
+      const auto& batches = c->host->batches;
+      if (!c->graph_exec || c->graph_gen != c->inputs_gen) {
+        if (c->graph_exec) {
+          (void)hipGraphExecDestroy(c->graph_exec);
+          c->graph_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        uint64_t batch_st = 0;
+        for (size_t i = 0; i < batches.size(); ++i) {
+          const uint64_t n = batches[i];
+          if (!n) continue;
+          c->host->Init((int)i, batch_st);
+          c->host->Launch((int)i, batch_st, batch_st + n, false);
+          c->host->Post((int)i, batch_st, n);
+          batch_st += n;
+        }
+        HIP_TRY(hipStreamEndCapture(s, &g));
+        const hipError_t ie = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(ie);
+        c->graph_gen = c->inputs_gen;
+      }
+      for (uint64_t n : batches) launches += n ? 1 : 0;
+      HIP_TRY(hipEventRecord(c->ev[3], s));
+      HIP_TRY(hipGraphLaunch(c->graph_exec, s));
+      HIP_TRY(hipEventRecord(c->ev[4], s));
+      HIP_TRY(hipEventSynchronize(c->ev[4]));
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
+      pre += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+      proc += ms;  // kernels and per-batch preparation inside the graph
+    } else {
     c->host->Init0();
     HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(hipEventSynchronize(c->ev[1]));
     HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     pre += ms;
     uint64_t batch_st = 0;
-    uint64_t launches = 0;
     const auto& batches = c->host->batches;
     for (size_t i = 0; i < batches.size(); ++i) {
       const uint64_t n = batches[i];
@@ -686,7 +759,7 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
       HIP_TRY(hipEventRecord(c->ev[2], s));
       c->host->Init((int)i, batch_st);
       HIP_TRY(hipEventRecord(c->ev[3], s));
-      c->host->Launch((int)i, batch_st, batch_st + n);
+      c->host->Launch((int)i, batch_st, batch_st + n, false);
       HIP_TRY(hipEventRecord(c->ev[4], s));
       c->host->Post((int)i, batch_st, n);
       HIP_TRY(hipEventRecord(c->ev[5], s));
@@ -699,6 +772,7 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
       proc += ms;
       batch_st += n;
       launches++;
+    }
     }
     HIP_TRY(hipEventRecord(c->ev[6], s));
     HIP_TRY(hipEventSynchronize(c->ev[6]));
@@ -777,6 +851,20 @@ int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) {
     return set_error(GCCTB_EKERNEL, "kernel error flags " + std::to_string(m.error_flags) + ":" + what);
   }
   if (m.committed != c->ntx) return set_error(GCCTB_EKERNEL, "not every transaction committed");
+  return GCCTB_OK;
+}
+
+int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FORM_PLAIN); }
+int gcctb_run_graph(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FORM_GRAPH); }
+int gcctb_run_persistent(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FORM_PERSIST); }
+
+int gcctb_set_batches(gcctb_ctx* c, const uint64_t* sizes, uint64_t n) {
+  if (!c || (n && !sizes)) return set_error(GCCTB_EINVAL, "null argument");
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < n; ++i) total += sizes[i];
+  if (n && total != c->ntx) return set_error(GCCTB_EINVAL, "batch sizes must add up to the loaded transactions");
+  c->custom_batches.assign(sizes, sizes + n);
+  c->inputs_gen++;
   return GCCTB_OK;
 }
 

@@ -43,7 +43,8 @@ typedef struct gcctb_config {
   uint64_t seed;           /* seeds the table and the written values */
   uint32_t backoff_ticks;  /* minimum unit of the randomized retry backoff (10 ns ticks); the unit
                               is the failed attempt's duration when that is longer */
-  uint32_t reserved;
+  uint32_t persist_window; /* persistent form: at most this many transactions in flight
+                              (0 = as many as the device holds) */
 } gcctb_config;
 
 /* Result of one run.  The first block mirrors the reference's Metrics and
@@ -104,6 +105,24 @@ int gcctb_load_ops(gcctb_ctx* ctx, const uint64_t* ops, const uint64_t* tx_off, 
  * benchmark/ycsb.cu:21-111).  The table keeps the result; run again to
  * continue on it. */
 int gcctb_run_plain(gcctb_ctx* ctx, gcctb_stats* out);
+
+/* The same kernels with the per-batch sequence captured once into a hipGraph
+ * and replayed: no host round trip between batches.  Results follow the same
+ * per-batch semantics as gcctb_run_plain. */
+int gcctb_run_graph(gcctb_ctx* ctx, gcctb_stats* out);
+
+/* Persistent batched form: the loaded transactions are one stream (batches
+ * one after another); one preparation pass and one persistent launch whose
+ * groups take the next transaction of the stream as soon as the current one
+ * commits.  Timestamps and tickets count across batch boundaries, so the
+ * serialization order spans the stream; GaccO / GPUTx results equal those of
+ * the plain form (transaction-id / (rank, id) serial order). */
+int gcctb_run_persistent(gcctb_ctx* ctx, gcctb_stats* out);
+
+/* Uneven batches for the plain and graph forms: sizes must add up to the
+ * loaded transaction count (replaces the uniform split of Init0,
+ * include/cc.cuh:29-35).  Loading new ops clears it. */
+int gcctb_set_batches(gcctb_ctx* ctx, const uint64_t* sizes, uint64_t n);
 
 /* Copy results back for checking.  Any pointer may be null.
  *   table:  rows*cols cells

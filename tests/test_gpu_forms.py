@@ -1,0 +1,76 @@
+"""GPU parity of the graph-captured and persistent forms, and of uneven
+batches: the same serial-replay check as the plain form, plus equality of the
+deterministic schemes' results across forms."""
+import numpy as np
+import pytest
+
+import gpu_cc_tb_amd as g
+import oracle_lib as orc
+from gpu_cc_tb_amd import workloads as wl
+
+FORM_CASES = {
+    # name: (rows, ntx, len_min, len_max, theta, read_ratio, batch, wd, bs)
+    "uniform": (4096, 2048, 16, 16, 0.0, 0.9, 512, 3, 8),
+    "hot": (64, 1024, 8, 8, 0.99, 0.5, 256, 4, 8),
+    "varlen": (1024, 1000, 4, 32, 0.8, 0.5, 333, 5, 16),
+}
+
+
+def check_replay(w, res, rows, seed):
+    init = orc.seed_table(seed, rows, 10)
+    exp_t, exp_r, ties = orc.replay(rows, 10, seed, init, w.ops, w.tx_off, res["order"])
+    assert ties == 0, "serialization keys must be distinct"
+    isr = ~g.op_is_write(w.ops)
+    assert np.array_equal(res["reads"][isr], exp_r[isr]), "reads differ from the serial replay"
+    assert np.array_equal(res["table"], exp_t), "final table differs from the serial replay"
+    return exp_t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["graph", "persistent"])
+@pytest.mark.parametrize("case", list(FORM_CASES))
+@pytest.mark.parametrize("scheme", list(g.SCHEMES))
+def test_form_parity(scheme, case, form):
+    rows, ntx, lmin, lmax, theta, rr, batch, wd, bs = FORM_CASES[case]
+    print(f"[forms] scheme={scheme} case={case} form={form}", flush=True)
+    w = g.generate(rows, ntx, theta=theta, read_ratio=rr, len_min=lmin, len_max=lmax)
+    with g.Context(scheme, rows=rows, warp_density=wd, block_size=bs, batch_size=batch) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run(form)
+        res = ctx.fetch(aborts=True)
+        cc = ctx.fetch_cc() if scheme not in ("gacco", "gputx") else None
+    assert st["committed"] == w.ntx and st["error_flags"] == 0
+    assert int(res["aborts"].sum()) == st["aborts"]
+    table = check_replay(w, res, rows, g.SEED)
+    if scheme == "gacco":
+        t2, _ = orc.replay_txid(rows, 10, g.SEED, orc.seed_table(g.SEED, rows, 10), w.ops, w.tx_off)
+        assert np.array_equal(table, t2)
+    if cc is not None:
+        if scheme in ("tpl_nw", "tpl_wd"):
+            assert not np.any(cc), "every lock is free (0) after the run"
+        else:
+            assert not np.any(cc), "touched CC words are reset after the run"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", list(g.SCHEMES))
+def test_uneven_batch_stream(scheme):
+    s = wl.batched_stream(2048, 12, 16, 400, len_lo=4, len_hi=32, theta_lo=0.0, theta_hi=0.99)
+    print(f"[stream] scheme={scheme} ntx={s.ntx} batches={s.batch_sizes}", flush=True)
+    results = {}
+    for form in ("plain", "graph", "persistent"):
+        with g.Context(scheme, rows=2048, warp_density=3, block_size=4, batch_size=1 << 20) as ctx:
+            ctx.seed_table()
+            ctx.load_ops(s.ops)
+            ctx.set_batches(s.batch_sizes)
+            st = ctx.run(form)
+            res = ctx.fetch()
+        assert st["committed"] == s.ntx and st["error_flags"] == 0
+        if form != "persistent":
+            assert st["batches"] == len(s.batch_sizes) or scheme == "gputx"
+        results[form] = check_replay(s.ops, res, 2048, g.SEED)
+    if scheme in ("gacco", "gputx"):
+        # deterministic schemes: every form gives the same table
+        assert np.array_equal(results["plain"], results["graph"])
+        assert np.array_equal(results["plain"], results["persistent"])
