@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark: committed transactions per second of the YCSB transaction path on
-one MI355X, under the eight CC schemes (tpl_nw tpl_wd to mvcc silo tictoc gacco
-gputx).
+one MI355X, batched (persistent) form against the plain per-batch form, under
+the eight CC schemes (tpl_nw tpl_wd to mvcc silo tictoc gacco gputx), plus the
+p50/p99 commit latency of the serving form.
 
-Workload (default): 10,485,760 rows x 10 u64 columns seeded from (seed, row,
-col); 1,048,576 transactions of 16 ops with Zipf(theta 0.1) keys, distinct per
-transaction; read ratio 0.9; batch_size 65,536 -- the "plain method on GPU, low
-contention" configuration.  Synthetic data, generated deterministically.
+Workload (throughput): 10,485,760 rows x 10 u64 columns seeded from (seed,
+row, col); 1,048,576 transactions of 16 ops with Zipf(theta 0.1) keys,
+distinct per transaction; read ratio 0.9; batch_size 65,536.  Synthetic data,
+generated deterministically.
 
-A step is one pass of every scheme over all 1,048,576 transactions with the
-reference's launch structure (per batch: reset/prepare, one kernel launch,
-synchronize).  The timed region holds exactly --steps steps; inputs are already
-resident in HBM.  `value` is whole-job committed Tx/s (8 x 1,048,576 x steps
-transactions / time).
+A step is one pass of every scheme over all 1,048,576 transactions in the
+batched form: one probe pass, one persistent launch over the 16 batches whose
+groups refill as transactions commit, one restore of the touched CC words.
+The timed region holds exactly --steps steps; inputs are already resident in
+HBM.  `value` is whole-job committed Tx/s of the batched form (8 x 1,048,576 x
+steps transactions / time).  The plain form (the reference's per batch
+reset + launch + synchronize) is timed the same way and reported beside it.
+
+Latency: the serving form on a 67,108,864-row table with requests of 16 ops,
+theta 0.9, read ratio 0.8, in batches of log-uniform size in [256, 65,536]
+arriving as a seeded Poisson process at a fraction of the measured peak.
 
 Prints one JSON line.
 """
@@ -30,14 +37,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import gpu_cc_tb_amd as g  # noqa: E402
+from gpu_cc_tb_amd import autotune, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--rows", type=int, default=10485760)
     p.add_argument("--txs", type=int, default=1048576)
@@ -46,24 +55,37 @@ def parse():
     p.add_argument("--read-ratio", type=float, default=0.9)
     p.add_argument("--batch", type=int, default=65536)
     p.add_argument("--schemes", default=",".join(g.SCHEMES))
-    p.add_argument("--wd", type=int, default=-1, help="warp_density; -1 = tuned default per scheme")
-    p.add_argument("--bs", type=int, default=-1, help="block_size; -1 = tuned default per scheme")
-    p.add_argument("--cpu-sample", type=int, default=262144, help="transactions per scheme for the CPU baseline")
+    p.add_argument("--wd", type=int, default=-1, help="warp_density; -1 = autotuned per scheme")
+    p.add_argument("--bs", type=int, default=-1, help="block_size; -1 = autotuned per scheme")
+    p.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    p.add_argument("--serve-rows", type=int, default=67108864)
+    p.add_argument("--serve-txs", type=int, default=262144, help="requests per serving run")
+    p.add_argument("--serve-load", type=float, default=0.5, help="offered load, fraction of measured peak")
+    p.add_argument("--serve-schemes", default="mvcc,silo")
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--cpu-sample", type=int, default=1048576, help="transactions per scheme for the CPU baseline")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
-
-
-# launch shape per scheme (warp_density, block_size): best of the wd x bs sweep
-# on this workload (wd 1..4 x bs 4, 8) on MI355X
-DEFAULT_SHAPE = {"tpl_nw": (3, 8), "tpl_wd": (3, 8), "to": (4, 8), "mvcc": (4, 4), "silo": (3, 8),
-                 "tictoc": (3, 8), "gacco": (2, 8), "gputx": (3, 8)}
 
 
 def algorithmic_bytes(ntx: int, nops: int, nreads: int) -> int:
     # per transaction 8 B header; per op 8 B op word + 8 B cell + 16 B CC word
     # load/store + 16 B index leaf entry; per read 8 B result
     return 8 * ntx + 48 * nops + 8 * nreads
+
+
+def traffic_per_launch():
+    """HBM bytes per launch of the execution kernel on this workload, from the
+    committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), or None when no summary is present."""
+    path = os.path.join(PROFILES, "txn_exec_config2_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1].strip().split()[0])
+    return None
 
 
 def cpu_baseline(w_full: g.Ops, args, schemes):
@@ -96,6 +118,44 @@ def cpu_baseline(w_full: g.Ops, args, schemes):
     }
 
 
+def run_form(ctx, form: str, steps: int):
+    tot = kern = 0.0
+    launches = aborts = 0
+    for _ in range(steps):
+        st = ctx.run(form)
+        tot += st["tot_ms"]
+        kern += st["exec_kernel_ms"]
+        launches += st["batches"]
+        aborts += st["aborts"]
+    return tot, kern, launches, aborts
+
+
+def serving(args, table):
+    """p50/p99 commit latency of the serving form at --serve-load of the
+    measured peak (peak = every request present at once)."""
+    # bounded sample: the first batches holding about --serve-txs requests
+    s = workloads.serving_stream(args.serve_rows, 4096, 256, 65536, theta=0.9, read_ratio=0.8,
+                                 max_txs=args.serve_txs)
+    sizes, ops, ntx, nb = s.batch_sizes, s.ops, s.ntx, len(s.batch_sizes)
+    out = {}
+    for scheme in [x for x in args.serve_schemes.split(",") if x]:
+        shape = autotune.lookup(scheme, 0.9, 0.8, table)
+        with g.Context(scheme, rows=args.serve_rows, warp_density=shape["warp_density"],
+                       block_size=shape["block_size"], batch_size=1 << 30,
+                       persist_window=shape["window"]) as ctx:
+            ctx.seed_table()
+            ctx.load_ops(ops)
+            ctx.serve(sizes)  # warm-up
+            peak = ctx.serve(sizes)
+            arr = workloads.poisson_arrivals(sizes, args.serve_load * peak["txps"])
+            st = ctx.serve(sizes, arr, max_launch_txs=262144)
+        out[scheme] = {"load": args.serve_load, "peak_txps": round(peak["txps"]), "txps": round(st["txps"]),
+                       "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                       "max_ms": round(st["max_ms"], 3), "launches": st["launches"], "txs": ntx,
+                       "batches": nb}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -103,72 +163,70 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     schemes = [s for s in args.schemes.split(",") if s]
+    table = autotune.load_table()
     t0 = time.time()
     w = g.generate(args.rows, args.txs, theta=args.theta, read_ratio=args.read_ratio, len_min=args.ops)
     nreads = int(np.count_nonzero(~g.op_is_write(w.ops)))
     gen_s = time.time() - t0
     info = g.device_info()
     per = {}
-    tot_ms = 0.0
-    kern_ms = 0.0
-    launches = 0
+    b_ms = b_kern = p_ms = 0.0
+    b_launch = 0
     for s in schemes:
-        wd, bs = DEFAULT_SHAPE[s]
-        if args.wd >= 0:
-            wd = args.wd
-        if args.bs >= 0:
-            bs = args.bs
-        with g.Context(s, rows=args.rows, warp_density=wd, block_size=bs, batch_size=args.batch) as ctx:
+        shape = autotune.lookup(s, args.theta, args.read_ratio, table)
+        wd = args.wd if args.wd >= 0 else shape["warp_density"]
+        bs = args.bs if args.bs >= 0 else shape["block_size"]
+        with g.Context(s, rows=args.rows, warp_density=wd, block_size=bs, batch_size=args.batch,
+                       persist_window=shape["window"]) as ctx:
             ctx.seed_table()
             ctx.load_ops(w)
             for _ in range(args.warmup):
-                ctx.run_plain()
-            s_ms = s_kern = 0.0
-            s_launch = s_abort = 0
-            wall0 = time.perf_counter()
-            for _ in range(args.steps):
-                st = ctx.run_plain()
-                s_ms += st["tot_ms"]
-                s_kern += st["exec_kernel_ms"]
-                s_launch += st["batches"]
-                s_abort += st["aborts"]
-            wall = time.perf_counter() - wall0
+                ctx.run("persistent")
+                if not args.no_plain:
+                    ctx.run("plain")
+            sb_ms, sb_kern, sb_launch, sb_abort = run_form(ctx, "persistent", args.steps)
+            sp_ms = 0.0
+            if not args.no_plain:
+                sp_ms, _, _, _ = run_form(ctx, "plain", args.steps)
         per[s] = {
-            "txps": round(args.txs * args.steps / (s_ms / 1e3)),
-            "wall_txps": round(args.txs * args.steps / wall),
-            "ms_per_pass": round(s_ms / args.steps, 3),
-            "kernel_ms_per_pass": round(s_kern / args.steps, 3),
-            "aborts_per_pass": s_abort // args.steps,
-            "launches_per_pass": s_launch // args.steps,
-            "wd": wd, "bs": bs,
+            "batched_txps": round(args.txs * args.steps / (sb_ms / 1e3)),
+            "plain_txps": round(args.txs * args.steps / (sp_ms / 1e3)) if sp_ms else None,
+            "batched_ms_per_pass": round(sb_ms / args.steps, 3),
+            "kernel_ms_per_pass": round(sb_kern / args.steps, 3),
+            "aborts_per_pass": sb_abort // args.steps,
+            "wd": wd, "bs": bs, "window": shape["window"],
         }
-        tot_ms += s_ms
-        kern_ms += s_kern
-        launches += s_launch
+        b_ms += sb_ms
+        b_kern += sb_kern
+        b_launch += sb_launch
+        p_ms += sp_ms
     committed = args.txs * args.steps * len(schemes)
-    value = committed / (tot_ms / 1e3)
-    bytes_total = algorithmic_bytes(args.txs, w.nops, nreads) * args.steps * len(schemes)
-    achieved = bytes_total / (kern_ms / 1e3) / 1e9
+    value = committed / (b_ms / 1e3)
+    bytes_launch = algorithmic_bytes(args.txs, w.nops, nreads)
+    avg_launch_ms = b_kern / max(1, b_launch)
+    achieved = bytes_launch / (avg_launch_ms / 1e3) / 1e9
     line = {
-        "metric": "committed Tx/s on 1 MI355X (plain per-batch form, 8 CC schemes)",
+        "metric": "committed Tx/s on 1 MI355X (batched vs plain) + p99 commit latency",
         "value": round(value),
         "unit": "Tx/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(tot_ms / args.steps, 3),
+        "ms_per_step": round(b_ms / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (deterministic Zipf YCSB generator, seeded table)",
         "config": {
-            "workload": f"ycsb plain: {args.rows} rows x 10 u64, {args.txs} tx x {args.ops} ops, "
+            "workload": f"ycsb low contention, batched form: {args.rows} rows x 10 u64, {args.txs} tx x {args.ops} ops, "
                         f"theta {args.theta}, read ratio {args.read_ratio}, batch {args.batch}, "
                         f"schemes {'+'.join(schemes)}",
             "rows": args.rows, "txs": args.txs, "ops_per_tx": args.ops, "theta": args.theta,
             "read_ratio": args.read_ratio, "batch_size": args.batch, "schemes": schemes,
         },
+        "plain": {"value": round(committed / (p_ms / 1e3)) if p_ms else None,
+                  "batched_over_plain": round(p_ms / b_ms, 3) if p_ms else None},
         "per_scheme": per,
         "roofline": {
             "bound": "hbm",
@@ -176,18 +234,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
-            "kernel": "txn_exec (all schemes)",
-            "bytes_per_tx": round(algorithmic_bytes(args.txs, w.nops, nreads) / args.txs, 1),
-            "avg_launch_ms": round(kern_ms / max(1, launches), 4),
+            "traffic": traffic_per_launch(),
+            "kernel": "txn_exec, persistent form (one launch per scheme per pass)",
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": round(avg_launch_ms, 4),
         },
         "device": info["name"],
         "gen_s": round(gen_s, 2),
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w, args, schemes)
-    else:
-        line["cpu_baseline"] = None
+    if not args.no_serve:
+        line["latency"] = serving(args, table)
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(w, args, schemes)
     print(json.dumps(line), flush=True)
 
 

@@ -39,6 +39,7 @@ class Config(C.Structure):
         ("seed", C.c_uint64),
         ("backoff_ticks", C.c_uint32),
         ("persist_window", C.c_uint32),
+        ("persist_adaptive", C.c_int32),
     ]
 
 
@@ -52,6 +53,22 @@ class Stats(C.Structure):
         ("batches", C.c_uint64), ("ranks", C.c_uint64), ("same_rank_conflicts", C.c_uint64),
         ("index_miss", C.c_uint64), ("error_flags", C.c_uint64), ("total_ops", C.c_uint64),
         ("read_ops", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("max_launch_txs", C.c_uint64), ("pool_cap_bytes", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("txs", C.c_uint64), ("committed", C.c_uint64), ("aborts", C.c_uint64), ("launches", C.c_uint64),
+        ("deferred", C.c_uint64), ("error_flags", C.c_uint64), ("tot_ms", C.c_double), ("busy_ms", C.c_double),
+        ("txps", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+        ("mean_ms", C.c_double), ("mean_launch_txs", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -94,6 +111,7 @@ def load_library() -> C.CDLL:
         "gcctb_run_graph": (C.c_int, [P, C.POINTER(Stats)]),
         "gcctb_run_persistent": (C.c_int, [P, C.POINTER(Stats)]),
         "gcctb_set_batches": (C.c_int, [P, _u64p, C.c_uint64]),
+        "gcctb_serve": (C.c_int, [P, C.POINTER(ServeConfig), _u64p, P, C.c_uint64, P, C.POINTER(ServeStats)]),
         "gcctb_fetch": (C.c_int, [P, P, P, P, P]),
         "gcctb_fetch_ranks": (C.c_int, [P, P, P, C.POINTER(C.c_uint64)]),
         "gcctb_fetch_cc": (C.c_int, [P, _u64p]),
@@ -222,12 +240,12 @@ class Context:
 
     def __init__(self, cc_scheme: str, *, rows: int, cols: int = 10, warp_density: int = 5,
                  block_size: int = 32, batch_size: int = 65536, seed: int = SEED,
-                 backoff_ticks: int = 20, persist_window: int = 0):
+                 backoff_ticks: int = 20, persist_window: int = 0, persist_adaptive: bool = False):
         self._lib = load_library()
         self.scheme = cc_scheme
         self.rows, self.cols, self.seed = rows, cols, seed
         cfg = Config(cc_scheme.encode(), warp_density, block_size, batch_size, rows, cols, seed,
-                     backoff_ticks, persist_window)
+                     backoff_ticks, persist_window, 1 if persist_adaptive else 0)
         h = C.c_void_p()
         _check(self._lib.gcctb_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -291,6 +309,29 @@ class Context:
 
     def run_plain(self, check: bool = True) -> dict:
         return self.run("plain", check)
+
+    def serve(self, batch_sizes, arrival_ms=None, *, max_launch_txs: int = 0, pool_cap_bytes: int = 0,
+              latencies: bool = False, check: bool = True) -> dict:
+        """Serving form: the loaded transactions arrive as batches (sizes in
+        order) at arrival_ms after the call starts (None: all at once).
+        Returns the serving statistics; with latencies=True also the
+        per-transaction commit latency (ms) under "latency_ms"."""
+        sizes = np.ascontiguousarray(batch_sizes, dtype=np.uint64)
+        arr = None if arrival_ms is None else np.ascontiguousarray(arrival_ms, dtype=np.float64)
+        if arr is not None and arr.size != sizes.size:
+            raise ValueError("one arrival time per batch")
+        lat = np.zeros(max(1, self.ntx), dtype=np.float64) if latencies else None
+        cfg = ServeConfig(max_launch_txs, pool_cap_bytes)
+        st = ServeStats()
+        rc = self._lib.gcctb_serve(self._h, C.byref(cfg), sizes if sizes.size else np.zeros(1, np.uint64),
+                                   _ptr(arr), sizes.size, _ptr(lat), C.byref(st))
+        if check:
+            _check(rc)
+        d = st.as_dict()
+        d["rc"] = rc
+        if latencies:
+            d["latency_ms"] = lat[: self.ntx]
+        return d
 
     def set_batches(self, sizes) -> None:
         sizes = np.ascontiguousarray(sizes, dtype=np.uint64)

@@ -1,0 +1,141 @@
+"""Launch-shape autotuner.
+
+Chooses, per scheme and contention bucket, the lanes per transaction
+(warp_density), the workgroup size (block_size, in 32-lane units as the
+reference counts them) and the persistent form's admission window
+(transactions in flight), by measuring the persistent form on a sample
+workload of that bucket.  The search is coordinate descent from the scheme's
+default shape: window first (the parameter that moves most with contention),
+then warp_density, then block_size, then the window once more.
+
+The result is a small JSON table keyed by scheme, then by bucket name
+"t<theta bucket>-r<read-ratio bucket>", each entry {"warp_density",
+"block_size", "window", "txps"}.  `lookup` maps a workload's skew and read
+ratio to the nearest measured bucket.
+
+    python -m gpu_cc_tb_amd.autotune --out gpu_cc_tb_amd/tuned.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+from . import SCHEMES, SEED, Context, generate
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE_PATH = os.path.join(HERE, "tuned.json")
+
+# skew and read-ratio buckets: representative value of each
+THETAS = {"0": 0.1, "1": 0.6, "2": 0.8, "3": 0.9, "4": 0.99}
+READS = {"w": 0.5, "r": 0.9, "o": 1.0}
+
+WD = [2, 3, 4, 5]
+BS = [4, 8, 16]
+WINDOWS = [0, 16384, 4096, 1024]
+
+# starting shapes (from sweeps of the plain form at low contention)
+DEFAULT_SHAPE = {"tpl_nw": (3, 8), "tpl_wd": (3, 8), "to": (4, 8), "mvcc": (4, 4), "silo": (3, 8),
+                 "tictoc": (3, 8), "gacco": (2, 8), "gputx": (3, 8)}
+
+
+def theta_bucket(theta: float) -> str:
+    return min(THETAS, key=lambda k: abs(THETAS[k] - theta))
+
+
+def read_bucket(rr: float) -> str:
+    return min(READS, key=lambda k: abs(READS[k] - rr))
+
+
+def bucket(theta: float, read_ratio: float) -> str:
+    return f"t{theta_bucket(theta)}-r{read_bucket(read_ratio)}"
+
+
+def load_table(path: str = TABLE_PATH) -> dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def lookup(scheme: str, theta: float, read_ratio: float, table: dict | None = None) -> dict:
+    """Tuned {warp_density, block_size, window} for a workload; the default
+    shape with a device-filling window when the table has no entry."""
+    table = load_table() if table is None else table
+    ent = table.get(scheme, {})
+    key = bucket(theta, read_ratio)
+    if key not in ent and ent:
+        # nearest measured skew with the same read bucket, else any
+        same_r = [k for k in ent if k.endswith("-r" + read_bucket(read_ratio))] or list(ent)
+        key = min(same_r, key=lambda k: abs(THETAS[k[1]] - theta))
+    if key in ent:
+        e = ent[key]
+        return {"warp_density": e["warp_density"], "block_size": e["block_size"], "window": e["window"]}
+    wd, bs = DEFAULT_SHAPE[scheme]
+    return {"warp_density": wd, "block_size": bs, "window": 0}
+
+
+def measure(scheme: str, rows: int, ops, batch_sizes, wd: int, bs: int, window: int, form: str) -> float:
+    """Committed transactions per second of one run (0 on a failed run)."""
+    with Context(scheme, rows=rows, warp_density=wd, block_size=bs, batch_size=1 << 30,
+                 persist_window=window) as c:
+        c.seed_table()
+        c.load_ops(ops)
+        c.set_batches(batch_sizes)
+        st = c.run(form, check=False)
+    if st["rc"] != 0 or st["committed"] != ops.ntx:
+        return 0.0
+    return ops.ntx / (st["tot_ms"] / 1e3)
+
+
+def tune(scheme: str, rows: int, ops, batch_sizes, form: str = "persistent", log=print) -> dict:
+    """Coordinate descent over window, warp_density, block_size."""
+    wd, bs = DEFAULT_SHAPE[scheme]
+    win = 0
+    cache: dict[tuple, float] = {}
+
+    def score(w, b, n):
+        k = (w, b, n)
+        if k not in cache:
+            t0 = time.time()
+            cache[k] = measure(scheme, rows, ops, batch_sizes, w, b, n, form)
+            log(json.dumps({"scheme": scheme, "wd": w, "bs": b, "window": n, "txps": round(cache[k]),
+                            "s": round(time.time() - t0, 2)}))
+        return cache[k]
+
+    windows = WINDOWS if form == "persistent" else [0]
+    win = max(windows, key=lambda n: score(wd, bs, n))
+    wd = max(WD, key=lambda w: score(w, bs, win))
+    bs = max(BS, key=lambda b: score(wd, b, win))
+    win = max(windows, key=lambda n: score(wd, bs, n))
+    return {"warp_density": wd, "block_size": bs, "window": win, "txps": round(score(wd, bs, win))}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--schemes", default=",".join(SCHEMES))
+    ap.add_argument("--thetas", default=",".join(THETAS))
+    ap.add_argument("--reads", default="w,r")
+    ap.add_argument("--rows", type=int, default=10485760)
+    ap.add_argument("--ntx", type=int, default=131072, help="sample transactions per bucket")
+    ap.add_argument("--batch", type=int, default=65536)
+    args = ap.parse_args(argv)
+    table = load_table(args.out)
+    for tb in args.thetas.split(","):
+        for rb in args.reads.split(","):
+            w = generate(args.rows, args.ntx, theta=THETAS[tb], read_ratio=READS[rb], seed=SEED)
+            sizes = [min(args.batch, args.ntx - s) for s in range(0, args.ntx, args.batch)]
+            for scheme in args.schemes.split(","):
+                best = tune(scheme, args.rows, w, sizes, log=lambda s: print(s, flush=True))
+                table.setdefault(scheme, {})[f"t{tb}-r{rb}"] = best
+                print(json.dumps({"scheme": scheme, "bucket": f"t{tb}-r{rb}", "best": best}), flush=True)
+                with open(args.out, "w") as f:
+                    json.dump(table, f, indent=1, sort_keys=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -269,6 +269,29 @@ struct TwoPL {
 // the data and clears the mark; abort restores the saved words.  The
 // serialization order is timestamp order.
 // ===========================================================================
+// After a cell of the version in `cur` (observed before the cell load) was
+// read: leave rts >= ts on that same version.  Other readers raising rts
+// meanwhile leave the version, and the value read, valid, so only a change of
+// the version bits sends the caller back to re-read (false; cur then holds the
+// current word).
+__device__ __forceinline__ bool hold_version(uint64_t* w, uint64_t ts, uint64_t& cur) {
+  const uint64_t ver = tsw::version(cur);
+  bool fresh = false;  // cur was observed after the cell load
+#pragma unroll 1
+  for (int t = 0; t < CAS_TRIES; ++t) {
+    if (fresh && tsw::version(cur) != ver) return false;
+    if (tsw::rts(cur) >= ts) {
+      if (fresh) return true;
+      cur = ld(w);
+      fresh = true;
+      continue;
+    }
+    if (cas(w, cur, tsw::make(0, ts, tsw::wts(cur)))) return true;
+    fresh = true;
+  }
+  return false;
+}
+
 struct TimestampOrdering {
   static __device__ StepResult read(uint64_t* w, uint64_t* cell, uint64_t ts, uint64_t& val, uint64_t& seen) {
     uint64_t cur = ld(w);
@@ -282,8 +305,7 @@ struct TimestampOrdering {
       order_acquire();
       val = ld(cell);
       order_acquire();
-      const uint64_t r = tsw::rts(cur) > ts ? tsw::rts(cur) : ts;
-      if (cas(w, cur, tsw::make(0, r, tsw::wts(cur)))) return R_OK;
+      if (hold_version(w, ts, cur)) return R_OK;
     }
     return R_WAIT;
   }
@@ -403,22 +425,32 @@ struct Mvcc {
                                     uint64_t& val) {
     uint64_t* w = a.cc + row;
     uint64_t* cell = a.cells + (uint64_t)row * a.cols + col;
+    uint64_t cur = ld(w);
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
-      uint64_t cur = ld(w);
       if (tsw::unc(cur)) return R_WAIT;
       if (tsw::wts(cur) <= ts) {
         order_acquire();
         val = ld(cell);
         order_acquire();
-        const uint64_t r = tsw::rts(cur) > ts ? tsw::rts(cur) : ts;
-        if (cas(w, cur, tsw::make(0, r, tsw::wts(cur)))) return R_OK;
+        if (hold_version(w, ts, cur)) return R_OK;
         continue;
       }
-      // walk back to the newest version with wts <= ts
+      // Walk back to the newest version with wts <= ts.  The head value and
+      // the chain head are read between two observations of the same version
+      // (a writer marks the word before it installs, so no install falls in
+      // between; rts raises by other readers do not matter).  Nodes below the
+      // chain head never change, so the walk itself needs no validation and a
+      // reader far behind a hot row's head still finishes.
       order_acquire();
-      uint64_t v = ld(cell);
       uint32_t p = ld32(a.vprev + row);
+      uint64_t v = ld(cell);
+      order_acquire();
+      const uint64_t after = ld(w);
+      if (tsw::version(after) != tsw::version(cur)) {
+        cur = after;
+        continue;
+      }
       uint64_t cur_wts = tsw::wts(cur);
       bool broken = false;
 #pragma unroll 1
@@ -433,8 +465,6 @@ struct Mvcc {
         cur_wts = tsw::wts(ld(node + 0));
         p = (uint32_t)link;
       }
-      order_acquire();
-      if (ld(w) != cur) continue;  // the head moved: retry (mvcc.cuh:196)
       if (broken) {
         atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_CHAIN_BROKEN);
         val = 0;
@@ -827,6 +857,10 @@ struct Ordered {
     if (c.ph == PH_START) {
       begin_attempt(c, now);
       c.ph = PH_OPS;
+      // each op's turn on its row, kept in registers for the polls
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (c.meta[k] & M_VALID) c.st[k] = a.need[c.op0 + op_index<G>(k)];
     }
     bool pending = false;
 #pragma unroll
@@ -834,9 +868,8 @@ struct Ordered {
       const uint32_t m = c.meta[k];
       if (!(m & M_VALID) || (m & M_DONE)) continue;
       const uint32_t j = op_index<G>(k);
-      const uint32_t need = a.need[c.op0 + j];
       uint32_t* ctr = a.counters + c.row[k];
-      if (ld32(ctr) < need) {
+      if (ld32(ctr) < (uint32_t)c.st[k]) {
         pending = true;
         continue;
       }
@@ -864,12 +897,8 @@ struct Ordered {
 //
 // Plain form (PERSIST = false): one launch per batch, one transaction per
 // group, the reference's launch structure (benchmark_cpu/ycsb.cu:262-285).
-// Persistent form (PERSIST = true): one launch over the whole stream with as
-// many groups as the device holds; group g runs slots g, g + NG, g + 2NG, ...
-// and loads its next transaction as soon as the current one commits, so no
-// group idles behind a batch's slowest transaction.  Each group takes its slots
-// in order, so the lowest unfinished slot is always resident, which the
-// ordered schemes (waits on lower slots only) need for progress.
+// Persistent form (PERSIST = true): one launch over the whole stream; groups
+// take the next transaction as soon as theirs commits (run_persistent).
 // ===========================================================================
 template <int G>
 __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t slot, uint64_t now) {
@@ -894,8 +923,13 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
     c.st[k] = 0;
     if (c.live && j < c.nops) {
       const uint64_t op = a.ops[c.op0 + j];
-      const uint32_t row = index_find(a.index, op_key(op));
-      a.rows_out[c.op0 + j] = row;
+      uint32_t row;
+      if (a.rows_in) {
+        row = a.rows_in[c.op0 + j];
+      } else {
+        row = index_find(a.index, op_key(op));
+        a.rows_out[c.op0 + j] = row;
+      }
       if (row < a.rows) {
         c.row[k] = row;
         c.meta[k] = M_VALID | (op_is_write(op) ? M_WRITE : 0u) | (op_col(op) << M_COL_SHIFT);
@@ -910,49 +944,210 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
   }
 }
 
-template <class S, int G, bool PERSIST>
-__global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
+// Watchdog stop: flag the error and, for diagnosis, dump the state of every op
+// of the group's unfinished transaction.
+template <int G>
+__device__ void watchdog_stop(const TxCtx<G>& c, const ExecArgs& a) {
   constexpr int K = TxCtx<G>::K;
-  TxCtx<G> c;
-  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
-  uint64_t slot = a.slot_lo + gtid / G;
-  c.t_round = now_ticks();
-  load_tx(c, a, slot, c.t_round);
+  if (lane_id() == 0) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_WATCHDOG);
+  if (a.dbg && !c.done) {
+    // per op: flags/row, the CC word through the coherent load and through an
+    // atomic read, and phase/attempts
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!(c.meta[k] & M_VALID)) continue;
+      uint64_t* d = a.dbg + (c.op0 + op_index<G>(k)) * 4;
+      uint64_t* w = a.cc + c.row[k];
+      d[0] = ((uint64_t)c.meta[k] << 32) | c.row[k];
+      d[1] = ld(w);
+      d[2] = fetch_or(w, 0);
+      d[3] = ((uint64_t)c.ph << 56) | ((uint64_t)c.attempts << 24) | 1u;
+    }
+  }
+}
+
+// Sum over the wave (a block can end in a half wave: lanes 0..31 only).
+__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
+  const bool full_wave = (ballot(true) >> 32) != 0;
+#pragma unroll
+  for (int m = 1; m < 32; m <<= 1) x += shfl_xor64(x, m);
+  if (full_wave) x += shfl_xor64(x, 32);
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent form.  Groups take stream slots, in stream order, from one
+// wave-aggregated counter as soon as their transaction commits, so a group
+// held up by a slow transaction never holds back later slots, and the lowest
+// unfinished slot always belongs to a running group (which the ordered
+// schemes need: they wait on lower slots only).
+//
+// Admission window W: a group whose rank is at or above W takes no new slot
+// (it parks).  Static window (adapt = 0): the grid itself holds W groups (or
+// as many as the device keeps resident), nobody parks.  Adaptive window
+// (adapt = 1): the grid fills the device and W moves with contention: waves
+// add their commits and aborted attempts to sharded counters every
+// FLUSH_TICKS; every CTL_TICKS one of a few controller waves compares the
+// aborted attempts per commit since the last update with two bounds: above 2,
+// W shrinks by a quarter (not below WIN_MIN); below 1/2, W grows by half plus
+// 64 (up to the grid's groups).  Group ranks interleave the blocks, so any
+// window spreads over the whole device.
+//
+// Exits: a group is finished when the counter has passed the end of the
+// stream; a wave leaves when all its groups are finished, when a live
+// transaction outlives the watchdog, or when it is parked and the slot
+// counter has not moved for the watchdog period.
+// ---------------------------------------------------------------------------
+constexpr uint64_t FLUSH_TICKS = 10000;  // 100 us of the 100 MHz wall clock
+constexpr uint64_t CTL_TICKS = 20000;    // 200 us
+constexpr uint64_t WIN_TICKS = 2000;     // a wave re-reads the window at most every 20 us
+constexpr uint64_t PARK_TICKS = 10000;   // parked groups re-check the stream every 100 us
+constexpr uint64_t WIN_MIN = 256;
+constexpr uint32_t CTL_BLOCK_STRIDE = 64;  // wave 0 of every 64th block may run the controller
+
+__device__ __forceinline__ void adapt_window(const ExecArgs& a, uint64_t now, uint64_t wmax) {
+  uint64_t* ctl = a.ctrs + CTR_CTL;
+  uint64_t due = ld(ctl);
+  if (now < due || !cas(ctl, due, now + CTL_TICKS)) return;
+  uint64_t cm = 0, ab = 0;
+#pragma unroll 1
+  for (int s = 0; s < STAT_SHARDS; ++s) {
+    cm += ld(a.ctrs + CTR_STAT0 + 16 * s);
+    ab += ld(a.ctrs + CTR_STAT0 + 16 * s + 1);
+  }
+  const uint64_t dc = cm - ld(ctl + 1), da = ab - ld(ctl + 2);
+  if (dc + da < 256) return;  // too few events to judge
+  st(ctl + 1, cm);
+  st(ctl + 2, ab);
+  uint64_t w = ld(a.ctrs + CTR_WIN);
+  if (w > wmax) w = wmax;
+  if (da > 2 * dc)
+    w = w - w / 4 < WIN_MIN ? WIN_MIN : w - w / 4;
+  else if (2 * da < dc)
+    w = w + w / 2 + 64 > wmax ? wmax : w + w / 2 + 64;
+  st(a.ctrs + CTR_WIN, w);
+}
+
+template <class S, int G>
+__device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups) {
+  const bool leader = (lane_id() & (G - 1)) == 0;
+  const uint64_t grank = (uint64_t)(threadIdx.x / G) * gridDim.x + blockIdx.x;
+  const uint64_t nslots = a.slot_hi - a.slot_lo;
+  const bool adapt = a.adapt != 0;
+  const bool controller = adapt && threadIdx.x < 64 && (blockIdx.x % CTL_BLOCK_STRIDE) == 0;
+  uint64_t* slot_ctr = a.ctrs + CTR_SLOT;
+  bool finished = false;
+  uint64_t acc_c = 0, acc_a = 0;
+  uint64_t now = now_ticks();
+  uint64_t next_flush = now + FLUSH_TICKS, next_ctl = now, next_park = now, next_w = now;
+  uint64_t wcache = adapt ? 0 : ~0ull;
+  uint64_t seen_slot = ~0ull, seen_at = now;
+  c.live = false;
+  c.done = true;
+  c.t_round = now;
 #pragma unroll 1
   while (true) {
-    if (PERSIST && c.done && slot < a.slot_hi) {
-      slot += ngroups;
-      if (slot < a.slot_hi) load_tx(c, a, slot, now_ticks());
+    // admission (the wall clock is a scalar read: `now` is wave-uniform)
+    const bool need = c.done && !finished;
+    bool stalled = false;
+    if (ballot(need) != 0) {
+      if (adapt && now >= next_w) {
+        wcache = ld(a.ctrs + CTR_WIN);
+        next_w = now + WIN_TICKS;
+      }
+      const bool take = need && grank < wcache;
+      if (ballot(take) != 0) {
+        const uint64_t got = group_bcast64<G>(wave_fetch_inc(slot_ctr, take && leader));
+        if (take) {
+          if (got < nslots)
+            load_tx(c, a, a.slot_lo + got, now_ticks());
+          else
+            finished = true;
+        }
+      }
+      if (ballot(need && !take) != 0 && now >= next_park) {
+        // parked groups: finished once the stream is handed out; a counter
+        // that has not moved for the watchdog period means no group progresses
+        next_park = now + PARK_TICKS;
+        const uint64_t cur = ld(slot_ctr);
+        if (cur >= nslots) {
+          if (need && !take) finished = true;
+        } else if (cur != seen_slot) {
+          seen_slot = cur;
+          seen_at = now;
+        } else if (a.watchdog_ticks && now - seen_at > a.watchdog_ticks) {
+          stalled = true;
+        }
+      }
     }
-    if (ballot(!c.done) == 0) break;
-    // every live group of the wave is backing off: sleep instead of spinning
-    if (ballot(!c.done && !(c.ph == PH_START && c.t_round < c.t_resume)) == 0)
+    if (ballot(!finished) == 0) break;
+    if (ballot(stalled) != 0) {
+      watchdog_stop(c, a);
+      break;
+    }
+    const uint32_t att0 = c.attempts;
+    const bool done0 = c.done;
+    if (ballot(!c.done) == 0)
+      __builtin_amdgcn_s_sleep(64);  // every group of the wave is parked
+    else if (ballot(!c.done && !(c.ph == PH_START && c.t_round < c.t_resume)) == 0)
       __builtin_amdgcn_s_sleep(4);
     else
       S::round(c, a);
-    c.t_round = now_ticks();
-    // Watchdog on the oldest live transaction of the wave (t_start is reset
-    // when a group loads a transaction); the wall clock is a scalar read, so
-    // the exit is wave-uniform.
-    const bool stuck = !c.done && a.watchdog_ticks && c.t_round - c.t_start > a.watchdog_ticks;
-    if (ballot(stuck) != 0) {
-      if (lane_id() == 0) atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_WATCHDOG);
-      if (a.dbg && !c.done) {
-        // per op of a stopped transaction: flags/row, the CC word through the
-        // coherent load and through an atomic read, and phase/attempts
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if (!(c.meta[k] & M_VALID)) continue;
-          uint64_t* d = a.dbg + (c.op0 + op_index<G>(k)) * 4;
-          uint64_t* w = a.cc + c.row[k];
-          d[0] = ((uint64_t)c.meta[k] << 32) | c.row[k];
-          d[1] = ld(w);
-          d[2] = fetch_or(w, 0);
-          d[3] = ((uint64_t)c.ph << 56) | ((uint64_t)c.attempts << 24) | 1u;
+    c.t_round = now = now_ticks();
+    if (adapt) {
+      if (leader) {
+        acc_a += c.attempts - att0;
+        acc_c += (!done0 && c.done) ? 1u : 0u;
+      }
+      if (now >= next_flush) {
+        next_flush = now + FLUSH_TICKS;
+        const uint64_t sc = wave_sum(acc_c), sa = wave_sum(acc_a);
+        acc_c = acc_a = 0;
+        if (lane_id() == 0 && (sc | sa)) {
+          uint64_t* stat = a.ctrs + CTR_STAT0 + 16 * (blockIdx.x & (STAT_SHARDS - 1));
+          if (sc) fetch_add(stat, sc);
+          if (sa) fetch_add(stat + 1, sa);
         }
       }
+      if (controller && now >= next_ctl) {
+        next_ctl = now + CTL_TICKS / 2;
+        if (lane_id() == 0) adapt_window(a, now, ngroups);
+      }
+    }
+    const bool stuck = !c.done && a.watchdog_ticks && now - c.t_start > a.watchdog_ticks;
+    if (ballot(stuck) != 0) {
+      watchdog_stop(c, a);
       break;
+    }
+  }
+}
+
+template <class S, int G, bool PERSIST>
+__global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
+  TxCtx<G> c;
+  if constexpr (PERSIST) {
+    run_persistent<S, G>(c, a, ((uint64_t)gridDim.x * blockDim.x) / G);
+  } else {
+    // plain form: one transaction per group
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    c.t_round = now_ticks();
+    load_tx(c, a, a.slot_lo + gtid / G, c.t_round);
+#pragma unroll 1
+    while (true) {
+      if (ballot(!c.done) == 0) break;
+      // every live group of the wave is backing off: sleep instead of spinning
+      if (ballot(!c.done && !(c.ph == PH_START && c.t_round < c.t_resume)) == 0)
+        __builtin_amdgcn_s_sleep(4);
+      else
+        S::round(c, a);
+      c.t_round = now_ticks();
+      // Watchdog on the oldest live transaction of the wave; the wall clock is
+      // a scalar read, so the exit is wave-uniform.
+      const bool stuck = !c.done && a.watchdog_ticks && c.t_round - c.t_start > a.watchdog_ticks;
+      if (ballot(stuck) != 0) {
+        watchdog_stop(c, a);
+        break;
+      }
     }
   }
   // metrics: sums over the transactions the group ran, reduced over the wave

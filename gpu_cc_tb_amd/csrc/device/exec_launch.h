@@ -24,8 +24,9 @@ hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream
     if (e != hipSuccess) return e;
     const uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 1) * (uint64_t)(cus > 0 ? cus : 1);
     if (blocks > resident) blocks = resident;
-    // admission window: fewer transactions in flight under heavy contention
-    if (a.persist_groups) {
+    // static admission window: fewer transactions in flight under heavy
+    // contention (the adaptive form keeps the whole grid and parks groups)
+    if (a.persist_groups && !a.adapt) {
       const uint64_t wblocks = (a.persist_groups * G + threads - 1) / threads;
       if (wblocks < blocks) blocks = wblocks < 1 ? 1 : wblocks;
     }

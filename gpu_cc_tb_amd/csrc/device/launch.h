@@ -85,6 +85,8 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
                       uint32_t* vprev, hipStream_t s);
 hipError_t batch_begin(uint64_t* ctrs, hipStream_t s);
+// adaptive admission state of a persistent launch (initial window in transactions)
+hipError_t adapt_begin(uint64_t* ctrs, uint64_t window, hipStream_t s);
 
 }  // namespace launch
 }  // namespace gcctb

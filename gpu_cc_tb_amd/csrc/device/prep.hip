@@ -452,6 +452,23 @@ __global__ void k_batch_begin(uint64_t* ctrs) {
   }
 }
 
+// Adaptive admission state for one persistent launch: slot counter, window,
+// controller words and the per-shard commit / abort counts.
+__global__ void k_adapt_begin(uint64_t* ctrs, uint64_t window) {
+  const int i = threadIdx.x;
+  if (i == 0) {
+    dev::st(ctrs + CTR_SLOT, 0);
+    dev::st(ctrs + CTR_WIN, window);
+    dev::st(ctrs + CTR_CTL, 0);
+    dev::st(ctrs + CTR_CTL + 1, 0);
+    dev::st(ctrs + CTR_CTL + 2, 0);
+  }
+  if (i < STAT_SHARDS) {
+    dev::st(ctrs + CTR_STAT0 + 16 * i, 0);
+    dev::st(ctrs + CTR_STAT0 + 16 * i + 1, 0);
+  }
+}
+
 }  // namespace
 
 namespace launch {
@@ -551,6 +568,11 @@ hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, ui
                       uint32_t* vprev, hipStream_t s) {
   GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, cc, vprev);
 }
+hipError_t adapt_begin(uint64_t* ctrs, uint64_t window, hipStream_t s) {
+  hipLaunchKernelGGL(k_adapt_begin, dim3(1), dim3(64), 0, s, ctrs, window);
+  return hipGetLastError();
+}
+
 hipError_t batch_begin(uint64_t* ctrs, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, s, ctrs);
   return hipGetLastError();

@@ -112,6 +112,9 @@ GC_HD uint64_t make(uint64_t u, uint64_t r, uint64_t wt) {
   return (u & 1u) | ((r & 0x7fffffffull) << 1) | ((wt & 0x7fffffffull) << 32);
 }
 constexpr uint64_t TS_MAX = 0x7fffffffull;  // 31-bit timestamps (to.cuh:108)
+// The version a word names: every bit but rts.  Placing a write mark and
+// publishing a version change these bits; raising rts never does.
+GC_HD uint64_t version(uint64_t w) { return w & ~(0x7fffffffull << 1); }
 }  // namespace tsw
 
 // Silo word (include/silo.cuh:17-32): bit 0 lock, bits 1..63 ts.
@@ -161,7 +164,14 @@ constexpr int CTR_POOL = 16;
 constexpr int CTR_TS0 = 32;
 constexpr int TS_SHARDS = 16;
 constexpr int TS_SHARD_BITS = 4;
-constexpr int CTR_WORDS = CTR_TS0 + 16 * TS_SHARDS;
+// Adaptive admission of the persistent form: next stream slot, admission
+// window, controller state, and per-shard commit / abort counts.
+constexpr int CTR_SLOT = CTR_TS0 + 16 * TS_SHARDS;
+constexpr int CTR_WIN = CTR_SLOT + 16;
+constexpr int CTR_CTL = CTR_WIN + 16;   // [0] next update tick, [1] commits, [2] aborts at the last update
+constexpr int CTR_STAT0 = CTR_CTL + 16;  // shard s: [16*s] commits, [16*s + 1] aborted attempts
+constexpr int STAT_SHARDS = 16;
+constexpr int CTR_WORDS = CTR_STAT0 + 16 * STAT_SHARDS;
 // Waves add their totals into one of METRIC_SLOTS 128-byte slots (by block
 // index) so that the end-of-kernel atomics do not pile onto one address; the
 // host sums the slots.
@@ -213,9 +223,10 @@ struct ExecArgs {
   uint64_t pool_cap;       // MVCC: node capacity
   uint32_t* counters;      // gacco/gputx: per-row completed-access counter
   const uint32_t* need;    // gacco/gputx: per-op access-group start
-  uint64_t* ctrs;          // [0] ts allocator, [1] ticket, [2] pool top
+  uint64_t* ctrs;          // ticket, pool top and timestamp shards (CTR_* word offsets)
   // outputs
   uint32_t* rows_out;      // per op: row the index returned
+  const uint32_t* rows_in; // per op: row from a probe pass before the launch (null: probe in the kernel)
   uint64_t* reads;         // per op: value returned by a read
   uint32_t* obatch;        // per tx: batch id
   uint64_t* okey0;         // per tx: serialization key (primary)
@@ -229,6 +240,7 @@ struct ExecArgs {
   uint64_t watchdog_ticks; // a wave gives up after this many ticks (0 = never)
   uint64_t* dbg;           // optional: 4 u64 per op, written for transactions the watchdog stopped
   uint64_t persist_groups; // persistent form: at most this many transactions in flight (0 = device-filling)
+  uint32_t adapt;          // persistent form: adapt the window to the abort rate (slots from CTR_SLOT)
 };
 
 }  // namespace gcctb

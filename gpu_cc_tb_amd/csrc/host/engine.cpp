@@ -5,10 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../device/launch.h"
@@ -119,6 +121,7 @@ struct gcctb_ctx {
   std::string dbg_path;
   uint64_t nranks = 0;
   uint64_t last_committed = 0;
+  bool rows_probed = false;  // rows_out holds every op's row (persistent form)
   std::vector<uint32_t> h_rank_counts;
   std::vector<uint64_t> custom_batches;  // uneven batch sizes (empty: batch_size)
   // hipGraph of the per-batch sequence (graph form), rebuilt when inputs change
@@ -154,6 +157,12 @@ class SchemeHost {
   virtual void PrepStream() { HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream)); }
   // Persistent form: restore the words every transaction of the stream touched.
   virtual void PostStream() { Post(0, 0, ctx.ntx); }
+  // Serving form: prepare one launch over transactions whose ops are [o0, o1).
+  virtual void PrepLaunch(uint64_t o0, uint64_t o1) {
+    (void)o0;
+    (void)o1;
+    HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream));
+  }
   virtual void Init(int batch_id, uint64_t batch_st) {
     (void)batch_id;
     (void)batch_st;
@@ -191,6 +200,7 @@ class SchemeHost {
     a.need = ctx.need.p;
     a.ctrs = ctx.ctrs.p;
     a.rows_out = ctx.rows_out.p;
+    a.rows_in = ctx.rows_probed ? ctx.rows_out.p : nullptr;
     a.reads = ctx.reads.p;
     a.obatch = ctx.obatch.p;
     a.okey0 = ctx.okey0.p;
@@ -204,6 +214,7 @@ class SchemeHost {
     a.watchdog_ticks = ctx.watchdog_ticks;
     a.dbg = ctx.dbg.p;
     a.persist_groups = ctx.cfg.persist_window;
+    a.adapt = ctx.cfg.persist_adaptive ? 1u : 0u;
     return a;
   }
   gcctb_ctx& ctx;
@@ -264,6 +275,7 @@ class GaccoHost : public SchemeHost {
   // the same as batch after batch.
   void PrepStream() override { prep_range(0, ctx.nops); }
   void PostStream() override {}
+  void PrepLaunch(uint64_t o0, uint64_t o1) override { prep_range(o0, o1); }
   // per-row access queues for ops [o0, o1) in (row, transaction id) order
   void prep_range(uint64_t o0, uint64_t o1) {
     const uint64_t n = o1 - o0;
@@ -675,6 +687,45 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
 }
 
 enum RunForm { FORM_PLAIN = 0, FORM_GRAPH = 1, FORM_PERSIST = 2 };
+// Largest per-batch sequence the graph form captures (GPUTx inputs can have
+// one rank level per launch by the hundred thousand).
+constexpr uint64_t GRAPH_MAX_LAUNCHES = 4096;
+
+// Slot counter and first admission window of a persistent launch: the
+// configured window, or every group of the grid.
+static uint64_t initial_window(const gcctb_ctx* c) {
+  return c->cfg.persist_window ? (uint64_t)c->cfg.persist_window : ~0ull >> 1;
+}
+
+// Totals of the per-slot metrics the kernels accumulated since the last clear.
+static Metrics collect_metrics(gcctb_ctx* c) {
+  Metrics m{};
+  std::vector<Metrics> slots(METRIC_SLOTS);
+  HIP_TRY(hipMemcpy(slots.data(), c->metrics.p, sizeof(Metrics) * METRIC_SLOTS, hipMemcpyDeviceToHost));
+  for (const Metrics& x : slots) {
+    m.abort += x.abort;
+    m.ts_duration += x.ts_duration;
+    m.wait_duration += x.wait_duration;
+    m.abort_duration += x.abort_duration;
+    m.manager_duration += x.manager_duration;
+    m.tot_duration += x.tot_duration;
+    m.index_duration += x.index_duration;
+    m.committed += x.committed;
+    m.index_miss += x.index_miss;
+    m.error_flags |= x.error_flags;
+  }
+  return m;
+}
+
+static std::string error_text(uint64_t flags) {
+  std::string what;
+  if (flags & ERR_TS_OVERFLOW) what += " timestamp-overflow";
+  if (flags & ERR_POOL_OVERFLOW) what += " version-pool-full";
+  if (flags & ERR_CHAIN_BROKEN) what += " version-chain-broken";
+  if (flags & ERR_TXLEN) what += " transaction-too-long";
+  if (flags & ERR_WATCHDOG) what += " watchdog";
+  return "kernel error flags " + std::to_string(flags) + ":" + what;
+}
 
 // One pass over every loaded transaction.
 //   plain:      the reference's structure -- per batch Init, one launch, Post,
@@ -697,8 +748,16 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (form == FORM_PERSIST) {
       c->host->PrepStream();
+      // Every op's row in one probe pass (GPUTx's ranking already made it):
+      // a group loading its next transaction then reads rows instead of
+      // walking the index while the other groups of its wave wait.
+      if (c->scheme != SCHEME_GPUTX && c->nops)
+        HIP_TRY(launch::probe(c->ix, c->ops.p, c->nops, c->rows_out.p, s));
+      HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
+      c->rows_probed = true;
       c->host->Launch(0, 0, c->ntx, true);
+      c->rows_probed = false;
       HIP_TRY(hipEventRecord(c->ev[4], s));
       c->host->PostStream();
       HIP_TRY(hipEventRecord(c->ev[5], s));
@@ -714,6 +773,11 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       // Init0 may synchronize with the host (GPUTx ranks): it stays outside
       c->host->Init0();
       const auto& batches = c->host->batches;
+      uint64_t nlaunch = 0;
+      for (uint64_t n : batches) nlaunch += n ? 1 : 0;
+      if (nlaunch > GRAPH_MAX_LAUNCHES)
+        return set_error(GCCTB_EINVAL, "the graph form captures at most " + std::to_string(GRAPH_MAX_LAUNCHES) +
+                                           " launches; this input needs " + std::to_string(nlaunch));
       if (!c->graph_exec || c->graph_gen != c->inputs_gen) {
         if (c->graph_exec) {
           (void)hipGraphExecDestroy(c->graph_exec);
@@ -782,22 +846,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     st.proc_ms = proc;
     st.exec_kernel_ms = proc;
     st.batches = launches;
-    {
-      std::vector<Metrics> slots(METRIC_SLOTS);
-      HIP_TRY(hipMemcpy(slots.data(), c->metrics.p, sizeof(Metrics) * METRIC_SLOTS, hipMemcpyDeviceToHost));
-      for (const Metrics& x : slots) {
-        m.abort += x.abort;
-        m.ts_duration += x.ts_duration;
-        m.wait_duration += x.wait_duration;
-        m.abort_duration += x.abort_duration;
-        m.manager_duration += x.manager_duration;
-        m.tot_duration += x.tot_duration;
-        m.index_duration += x.index_duration;
-        m.committed += x.committed;
-        m.index_miss += x.index_miss;
-        m.error_flags |= x.error_flags;
-      }
-    }
+    m = collect_metrics(c);
     if (c->scheme == SCHEME_GPUTX) {
       st.ranks = c->nranks;
       if (c->nops) HIP_TRY(hipMemcpy(&st.same_rank_conflicts, c->conflicts.p, 8, hipMemcpyDeviceToHost));
@@ -824,6 +873,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       }
     }
   } catch (const HipFailure& f) {
+    c->rows_probed = false;
     return hip_fail(f);
   }
   st.txs = c->ntx;
@@ -841,15 +891,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
   st.read_ops = c->nreads;
   c->last_committed = m.committed;
   if (out) *out = st;
-  if (m.error_flags) {
-    std::string what;
-    if (m.error_flags & ERR_TS_OVERFLOW) what += " timestamp-overflow";
-    if (m.error_flags & ERR_POOL_OVERFLOW) what += " version-pool-full";
-    if (m.error_flags & ERR_CHAIN_BROKEN) what += " version-chain-broken";
-    if (m.error_flags & ERR_TXLEN) what += " transaction-too-long";
-    if (m.error_flags & ERR_WATCHDOG) what += " watchdog";
-    return set_error(GCCTB_EKERNEL, "kernel error flags " + std::to_string(m.error_flags) + ":" + what);
-  }
+  if (m.error_flags) return set_error(GCCTB_EKERNEL, error_text(m.error_flags));
   if (m.committed != c->ntx) return set_error(GCCTB_EKERNEL, "not every transaction committed");
   return GCCTB_OK;
 }
@@ -857,6 +899,161 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
 int gcctb_run_plain(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FORM_PLAIN); }
 int gcctb_run_graph(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FORM_GRAPH); }
 int gcctb_run_persistent(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FORM_PERSIST); }
+
+// ---------------------------------------------------------------------------
+// Serving form.  Requests arrive as batches at given times; table, index and
+// CC state stay resident on the device.  Whenever the device is idle the host
+// admits every batch that has arrived, up to the admission limits, copies
+// their ops from pinned host memory and runs them as one persistent launch
+// (probe pass, persistent kernel, restore of the touched CC words).  A
+// transaction's latency is its launch's completion time minus its batch's
+// arrival time, both on the host clock.
+// Admission: at most max_launch_txs transactions per launch, and, for MVCC,
+// at most as many writes as the version pool holds (the pool budget is
+// pool_cap_bytes, 24 bytes per version).  Batches that arrived but do not fit
+// wait for the next launch (counted in `deferred`).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PinnedBuf {
+  uint64_t* p = nullptr;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+double percentile(std::vector<double> v, double q) {
+  if (v.empty()) return 0;
+  const size_t k = std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5));
+  std::nth_element(v.begin(), v.begin() + (ptrdiff_t)k, v.end());
+  return v[k];
+}
+
+}  // namespace
+
+int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batch_sizes, const double* arrival_ms,
+                uint64_t nbatches, double* tx_latency_ms, gcctb_serve_stats* out) {
+  if (!c || !sc || (nbatches && !batch_sizes)) return set_error(GCCTB_EINVAL, "null argument");
+  if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
+  if (c->scheme == SCHEME_GPUTX)
+    return set_error(GCCTB_EINVAL, "gputx ranks its whole input before running; it has no serving form");
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < nbatches; ++i) {
+    if (batch_sizes[i] == 0) return set_error(GCCTB_EINVAL, "empty batch");
+    total += batch_sizes[i];
+    if (arrival_ms && (!(arrival_ms[i] >= 0) || (i && arrival_ms[i] < arrival_ms[i - 1])))
+      return set_error(GCCTB_EINVAL, "arrival times must be non-negative and non-decreasing");
+  }
+  if (total != c->ntx) return set_error(GCCTB_EINVAL, "batch sizes must add up to the loaded transactions");
+  const uint64_t max_txs = sc->max_launch_txs ? sc->max_launch_txs : c->ntx;
+  uint64_t pool_nodes = ~0ull;
+  if (c->scheme == SCHEME_MVCC) {
+    pool_nodes = c->pool_cap;
+    if (sc->pool_cap_bytes) pool_nodes = std::min(pool_nodes, sc->pool_cap_bytes / 24);
+  }
+  gcctb_serve_stats st{};
+  st.txs = c->ntx;
+  using clk = std::chrono::steady_clock;
+  const uint64_t saved_pool_cap = c->pool_cap;
+  try {
+    hipStream_t s = c->stream;
+    // requests wait in pinned host memory until admitted
+    PinnedBuf host_ops;
+    if (c->nops) {
+      HIP_TRY(hipHostMalloc((void**)&host_ops.p, c->nops * 8, hipHostMallocDefault));
+      HIP_TRY(hipMemcpy(host_ops.p, c->ops.p, c->nops * 8, hipMemcpyDeviceToHost));
+    }
+    // per-batch write counts (version-pool admission)
+    std::vector<uint64_t> bfirst(nbatches + 1, 0), bwrites(nbatches, 0);
+    for (uint64_t i = 0; i < nbatches; ++i) {
+      bfirst[i + 1] = bfirst[i] + batch_sizes[i];
+      for (uint64_t o = c->h_tx_off[bfirst[i]]; o < c->h_tx_off[bfirst[i + 1]]; ++o)
+        bwrites[i] += op_is_write(host_ops.p[o]) ? 1 : 0;
+      if (bwrites[i] > pool_nodes) return set_error(GCCTB_EINVAL, "a batch has more writes than the version pool holds");
+    }
+    HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics) * METRIC_SLOTS, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (c->scheme == SCHEME_MVCC) c->pool_cap = std::min(c->pool_cap, pool_nodes);
+    std::vector<double> lat(c->ntx, 0.0);
+    double busy = 0, launch_txs = 0;
+    const auto t0 = clk::now();
+    auto since = [&](clk::time_point t) { return std::chrono::duration<double, std::milli>(t - t0).count(); };
+    uint64_t next = 0, launches = 0;
+    while (next < nbatches) {
+      const double due = arrival_ms ? arrival_ms[next] : 0.0;
+      double now = since(clk::now());
+      if (due > now) {
+        if (due - now > 0.2) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)((due - now - 0.1) * 1e3)));
+        while (since(clk::now()) < due) {
+        }
+        now = since(clk::now());
+      }
+      // admit every arrived batch that fits
+      uint64_t end = next, txs = 0, writes = 0;
+      while (end < nbatches && (!arrival_ms || arrival_ms[end] <= now)) {
+        if (end > next && (txs + batch_sizes[end] > max_txs || writes + bwrites[end] > pool_nodes)) {
+          st.deferred++;
+          break;
+        }
+        txs += batch_sizes[end];
+        writes += bwrites[end];
+        ++end;
+      }
+      const uint64_t lo = bfirst[next], hi = bfirst[end];
+      const uint64_t o0 = c->h_tx_off[lo], o1 = c->h_tx_off[hi];
+      const auto tl = clk::now();
+      if (o1 > o0) HIP_TRY(hipMemcpyAsync(c->ops.p + o0, host_ops.p + o0, (o1 - o0) * 8, hipMemcpyHostToDevice, s));
+      c->host->PrepLaunch(o0, o1);
+      if (o1 > o0) HIP_TRY(launch::probe(c->ix, c->ops.p + o0, o1 - o0, c->rows_out.p + o0, s));
+      HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
+      c->rows_probed = true;
+      c->host->Launch((int)launches, lo, hi, true);
+      c->rows_probed = false;
+      c->host->Post((int)launches, lo, hi - lo);
+      HIP_TRY(hipEventRecord(c->ev[7], s));
+      HIP_TRY(hipEventSynchronize(c->ev[7]));
+      const auto td = clk::now();
+      const double done = since(td);
+      busy += std::chrono::duration<double, std::milli>(td - tl).count();
+      for (uint64_t b = next; b < end; ++b) {
+        const double arr = arrival_ms ? arrival_ms[b] : 0.0;
+        for (uint64_t t = bfirst[b]; t < bfirst[b + 1]; ++t) lat[t] = done - arr;
+      }
+      launch_txs += (double)txs;
+      ++launches;
+      next = end;
+    }
+    st.tot_ms = since(clk::now());
+    c->pool_cap = saved_pool_cap;
+    const Metrics m = collect_metrics(c);
+    st.committed = m.committed;
+    st.aborts = m.abort;
+    st.error_flags = m.error_flags;
+    st.launches = launches;
+    st.busy_ms = busy;
+    st.mean_launch_txs = launches ? launch_txs / (double)launches : 0;
+    st.txps = st.tot_ms > 0 ? (double)m.committed / (st.tot_ms / 1e3) : 0;
+    double sum = 0, mx = 0;
+    for (double x : lat) {
+      sum += x;
+      mx = std::max(mx, x);
+    }
+    st.mean_ms = lat.empty() ? 0 : sum / (double)lat.size();
+    st.max_ms = mx;
+    st.p50_ms = percentile(lat, 0.50);
+    st.p99_ms = percentile(lat, 0.99);
+    if (tx_latency_ms && !lat.empty()) std::memcpy(tx_latency_ms, lat.data(), lat.size() * sizeof(double));
+    c->last_committed = m.committed;
+  } catch (const HipFailure& f) {
+    c->rows_probed = false;
+    c->pool_cap = saved_pool_cap;
+    return hip_fail(f);
+  }
+  if (out) *out = st;
+  if (st.error_flags) return set_error(GCCTB_EKERNEL, error_text(st.error_flags));
+  if (st.committed != c->ntx) return set_error(GCCTB_EKERNEL, "not every transaction committed");
+  return GCCTB_OK;
+}
 
 int gcctb_set_batches(gcctb_ctx* c, const uint64_t* sizes, uint64_t n) {
   if (!c || (n && !sizes)) return set_error(GCCTB_EINVAL, "null argument");

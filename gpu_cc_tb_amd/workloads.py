@@ -71,3 +71,33 @@ def config4(seed: int = SEED, nbatches: int = 256, rows: int = 10485760, threads
 
 def uniform_batches(ntx: int, batch: int) -> list[int]:
     return [min(batch, ntx - s) for s in range(0, ntx, batch)]
+
+
+def serving_stream(rows: int = 67108864, nbatches: int = 256, size_lo: int = 256, size_hi: int = 65536, *,
+                   length: int = 16, theta: float = 0.9, read_ratio: float = 0.8, seed: int = SEED,
+                   threads: int = 0, max_txs: int | None = None) -> Stream:
+    """Serving requests: batch sizes log-uniform in [size_lo, size_hi], every
+    transaction `length` ops with one Zipf skew and read ratio.  With max_txs,
+    only the first batches holding at least max_txs transactions are kept."""
+    r = _rng(seed, 5)
+    sizes = np.exp(r.uniform(math.log(size_lo), math.log(size_hi), nbatches)).astype(np.int64)
+    sizes = [int(x) for x in np.clip(sizes, size_lo, size_hi)]
+    if max_txs is not None:
+        keep = int(np.searchsorted(np.cumsum(sizes), max_txs)) + 1
+        sizes = sizes[:keep]
+    ops = generate(rows, int(sum(sizes)), theta=theta, read_ratio=read_ratio, len_min=length, len_max=length,
+                   seed=seed, threads=threads)
+    return Stream(ops, sizes, [{"size": n, "theta": theta, "read_ratio": read_ratio} for n in sizes])
+
+
+def poisson_arrivals(batch_sizes, txps: float, seed: int = SEED) -> np.ndarray:
+    """Arrival times (ms) of a Poisson process of batches whose mean rate
+    carries `txps` transactions per second: exponential gaps with mean
+    mean(batch size) / txps."""
+    sizes = np.asarray(batch_sizes, dtype=np.float64)
+    if sizes.size == 0:
+        return np.zeros(0)
+    mean_gap_ms = 1e3 * float(sizes.mean()) / txps
+    gaps = _rng(seed, 6).exponential(mean_gap_ms, sizes.size)
+    gaps[0] = 0.0
+    return np.cumsum(gaps)

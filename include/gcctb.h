@@ -43,8 +43,10 @@ typedef struct gcctb_config {
   uint64_t seed;           /* seeds the table and the written values */
   uint32_t backoff_ticks;  /* minimum unit of the randomized retry backoff (10 ns ticks); the unit
                               is the failed attempt's duration when that is longer */
-  uint32_t persist_window; /* persistent form: at most this many transactions in flight
-                              (0 = as many as the device holds) */
+  uint32_t persist_window; /* persistent and serving forms: at most this many transactions in
+                              flight (0 = as many as the device holds) */
+  int32_t persist_adaptive; /* 1: the window adapts to the abort rate while the launch runs,
+                               starting from persist_window */
 } gcctb_config;
 
 /* Result of one run.  The first block mirrors the reference's Metrics and
@@ -108,7 +110,8 @@ int gcctb_run_plain(gcctb_ctx* ctx, gcctb_stats* out);
 
 /* The same kernels with the per-batch sequence captured once into a hipGraph
  * and replayed: no host round trip between batches.  Results follow the same
- * per-batch semantics as gcctb_run_plain. */
+ * per-batch semantics as gcctb_run_plain.  At most 4096 launches (batches, or
+ * GPUTx rank levels); more returns GCCTB_EINVAL. */
 int gcctb_run_graph(gcctb_ctx* ctx, gcctb_stats* out);
 
 /* Persistent batched form: the loaded transactions are one stream (batches
@@ -118,6 +121,35 @@ int gcctb_run_graph(gcctb_ctx* ctx, gcctb_stats* out);
  * serialization order spans the stream; GaccO / GPUTx results equal those of
  * the plain form (transaction-id / (rank, id) serial order). */
 int gcctb_run_persistent(gcctb_ctx* ctx, gcctb_stats* out);
+
+/* Serving form (no reference counterpart: the reference runs a fixed input
+ * file, benchmark_cpu/ycsb.cu:242-291).  The loaded transactions are requests
+ * that arrive in batches: batch i holds the next batch_sizes[i] transactions
+ * and arrives arrival_ms[i] after the call starts (null: every batch at 0, the
+ * peak-load case).  Table, index and CC state stay on the device; whenever it
+ * is idle, every arrived batch that fits the admission limits is copied from
+ * pinned host memory and run as one persistent launch.  tx_latency_ms (may be
+ * null) receives, per transaction, completion of its launch minus arrival of
+ * its batch.  Order keys from gcctb_fetch are {launch, key0, key1}.  Not
+ * available for gputx (it ranks its whole input before running). */
+typedef struct gcctb_serve_config {
+  uint64_t max_launch_txs;  /* admission: transactions per launch (0 = no limit) */
+  uint64_t pool_cap_bytes;  /* admission, MVCC: version-pool budget, 24 bytes per write
+                               (0 = the pool load_ops sized) */
+} gcctb_serve_config;
+typedef struct gcctb_serve_stats {
+  uint64_t txs, committed, aborts;
+  uint64_t launches;
+  uint64_t deferred;        /* times an arrived batch waited for a later launch */
+  uint64_t error_flags;
+  double tot_ms;            /* call start to the last completion */
+  double busy_ms;           /* sum of launch durations (copy, prepare, run, restore) */
+  double txps;              /* committed / tot_ms */
+  double p50_ms, p99_ms, max_ms, mean_ms;  /* commit latency */
+  double mean_launch_txs;
+} gcctb_serve_stats;
+int gcctb_serve(gcctb_ctx* ctx, const gcctb_serve_config* cfg, const uint64_t* batch_sizes,
+                const double* arrival_ms, uint64_t nbatches, double* tx_latency_ms, gcctb_serve_stats* out);
 
 /* Uneven batches for the plain and graph forms: sizes must add up to the
  * loaded transaction count (replaces the uniform split of Init0,

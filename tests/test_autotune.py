@@ -1,0 +1,23 @@
+"""Host logic of the autotuner's table (no GPU)."""
+from gpu_cc_tb_amd import autotune as at
+
+
+def test_buckets():
+    assert at.bucket(0.0, 0.9) == "t0-rr"
+    assert at.bucket(0.99, 0.5) == "t4-rw"
+    assert at.bucket(0.95, 1.0) == "t4-ro"
+    assert at.bucket(0.7, 0.8) in ("t1-rr", "t2-rr")
+
+
+def test_lookup_default_and_nearest():
+    assert at.lookup("silo", 0.9, 0.9, table={}) == {"warp_density": 3, "block_size": 8, "window": 0}
+    table = {"silo": {"t0-rr": {"warp_density": 4, "block_size": 16, "window": 0, "txps": 1},
+                      "t4-rr": {"warp_density": 3, "block_size": 8, "window": 1024, "txps": 1},
+                      "t4-rw": {"warp_density": 2, "block_size": 4, "window": 4096, "txps": 1}}}
+    assert at.lookup("silo", 0.99, 0.9, table)["window"] == 1024
+    assert at.lookup("silo", 0.99, 0.5, table)["window"] == 4096
+    # no t3 entry: nearest skew with the same read bucket
+    assert at.lookup("silo", 0.9, 0.9, table)["window"] == 1024
+    assert at.lookup("silo", 0.05, 0.9, table)["block_size"] == 16
+    # another scheme falls back to its default shape
+    assert at.lookup("gacco", 0.99, 0.9, table)["warp_density"] == 2

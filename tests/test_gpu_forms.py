@@ -74,3 +74,46 @@ def test_uneven_batch_stream(scheme):
         # deterministic schemes: every form gives the same table
         assert np.array_equal(results["plain"], results["graph"])
         assert np.array_equal(results["plain"], results["persistent"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("window,adaptive", [(64, False), (0, True), (256, True)])
+@pytest.mark.parametrize("case", ["hot", "varlen"])
+@pytest.mark.parametrize("scheme", list(g.SCHEMES))
+def test_persistent_window(scheme, case, window, adaptive):
+    """Persistent form under a static admission window and under the adaptive
+    window (slots from the shared counter, groups parked above the window):
+    serial-replay exact, every transaction committed once."""
+    rows, ntx, lmin, lmax, theta, rr, batch, wd, bs = FORM_CASES[case]
+    print(f"[window] scheme={scheme} case={case} window={window} adaptive={adaptive}", flush=True)
+    w = g.generate(rows, ntx, theta=theta, read_ratio=rr, len_min=lmin, len_max=lmax)
+    with g.Context(scheme, rows=rows, warp_density=wd, block_size=bs, batch_size=batch,
+                   persist_window=window, persist_adaptive=adaptive) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run("persistent")
+        res = ctx.fetch(aborts=True)
+    assert st["committed"] == w.ntx and st["error_flags"] == 0
+    assert int(res["aborts"].sum()) == st["aborts"]
+    check_replay(w, res, rows, g.SEED)
+
+
+@pytest.mark.gpu
+def test_graph_launch_cap():
+    """GPUTx with more rank levels than the graph form captures: the graph
+    form refuses with an error, the plain form runs it (one launch per level)."""
+    rows, ntx = 4, 10000
+    w = g.generate(rows, ntx, theta=0.0, read_ratio=0.0, len_min=2, len_max=2)
+    ranks, counts = orc.gputx_rank(rows, w.ops, w.tx_off)
+    nranks = int(ranks.max()) + 1
+    print(f"[graphcap] ranks={nranks}", flush=True)
+    assert nranks > 4096
+    with g.Context("gputx", rows=rows, warp_density=1, block_size=1, batch_size=ntx) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        with pytest.raises(g.GcctbError):
+            ctx.run("graph")
+        st = ctx.run("plain")
+        res = ctx.fetch()
+    assert st["committed"] == ntx and st["batches"] == nranks
+    check_replay(w, res, rows, g.SEED)
