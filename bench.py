@@ -142,7 +142,7 @@ def serving(args, table):
         shape = autotune.lookup(scheme, 0.9, 0.8, table)
         with g.Context(scheme, rows=args.serve_rows, warp_density=shape["warp_density"],
                        block_size=shape["block_size"], batch_size=1 << 30,
-                       persist_window=shape["window"]) as ctx:
+                       persist_window=shape["window"], persist_adaptive=shape["adaptive"]) as ctx:
             ctx.seed_table()
             ctx.load_ops(ops)
             ctx.serve(sizes)  # warm-up
@@ -177,7 +177,7 @@ def main():
         wd = args.wd if args.wd >= 0 else shape["warp_density"]
         bs = args.bs if args.bs >= 0 else shape["block_size"]
         with g.Context(s, rows=args.rows, warp_density=wd, block_size=bs, batch_size=args.batch,
-                       persist_window=shape["window"]) as ctx:
+                       persist_window=shape["window"], persist_adaptive=shape["adaptive"]) as ctx:
             ctx.seed_table()
             ctx.load_ops(w)
             for _ in range(args.warmup):

@@ -9,9 +9,10 @@ default shape: window first (the parameter that moves most with contention),
 then warp_density, then block_size, then the window once more.
 
 The result is a small JSON table keyed by scheme, then by bucket name
-"t<theta bucket>-r<read-ratio bucket>", each entry {"warp_density",
-"block_size", "window", "txps"}.  `lookup` maps a workload's skew and read
-ratio to the nearest measured bucket.
+"t<theta bucket>-r<read-ratio bucket>" (or "stream" for the batched stream
+of mixed skew), each entry {"warp_density", "block_size", "window",
+"adaptive", "txps"}.  `lookup` maps a workload's skew and read ratio to the
+nearest measured bucket.
 
     python -m gpu_cc_tb_amd.autotune --out gpu_cc_tb_amd/tuned.json
 """
@@ -32,9 +33,10 @@ TABLE_PATH = os.path.join(HERE, "tuned.json")
 THETAS = {"0": 0.1, "1": 0.6, "2": 0.8, "3": 0.9, "4": 0.99}
 READS = {"w": 0.5, "r": 0.9, "o": 1.0}
 
-WD = [2, 3, 4, 5]
+WD = [2, 3, 4]
 BS = [4, 8, 16]
-WINDOWS = [0, 16384, 4096, 1024]
+# admission windows (transactions in flight; 0 = device-filling, -1 = adaptive)
+WINDOWS = [0, 4096, 1024, 256, -1]
 
 # starting shapes (from sweeps of the plain form at low contention)
 DEFAULT_SHAPE = {"tpl_nw": (3, 8), "tpl_wd": (3, 8), "to": (4, 8), "mvcc": (4, 4), "silo": (3, 8),
@@ -60,27 +62,33 @@ def load_table(path: str = TABLE_PATH) -> dict:
         return json.load(f)
 
 
-def lookup(scheme: str, theta: float, read_ratio: float, table: dict | None = None) -> dict:
-    """Tuned {warp_density, block_size, window} for a workload; the default
-    shape with a device-filling window when the table has no entry."""
+def lookup(scheme: str, theta: float, read_ratio: float, table: dict | None = None, key: str | None = None) -> dict:
+    """Tuned {warp_density, block_size, window, adaptive} for a workload (or
+    for the named bucket `key`, e.g. "stream", when the table has it); the
+    default shape with a device-filling window when the table has no entry."""
     table = load_table() if table is None else table
     ent = table.get(scheme, {})
-    key = bucket(theta, read_ratio)
+    if key is None or key not in ent:
+        key = bucket(theta, read_ratio)
     if key not in ent and ent:
         # nearest measured skew with the same read bucket, else any
-        same_r = [k for k in ent if k.endswith("-r" + read_bucket(read_ratio))] or list(ent)
-        key = min(same_r, key=lambda k: abs(THETAS[k[1]] - theta))
+        buckets = [k for k in ent if k.startswith("t")]
+        same_r = [k for k in buckets if k.endswith("-r" + read_bucket(read_ratio))] or buckets
+        if same_r:
+            key = min(same_r, key=lambda k: abs(THETAS[k[1]] - theta))
     if key in ent:
         e = ent[key]
-        return {"warp_density": e["warp_density"], "block_size": e["block_size"], "window": e["window"]}
+        return {"warp_density": e["warp_density"], "block_size": e["block_size"], "window": e["window"],
+                "adaptive": bool(e.get("adaptive", False))}
     wd, bs = DEFAULT_SHAPE[scheme]
-    return {"warp_density": wd, "block_size": bs, "window": 0}
+    return {"warp_density": wd, "block_size": bs, "window": 0, "adaptive": False}
 
 
 def measure(scheme: str, rows: int, ops, batch_sizes, wd: int, bs: int, window: int, form: str) -> float:
-    """Committed transactions per second of one run (0 on a failed run)."""
+    """Committed transactions per second of one run (0 on a failed run);
+    window -1 selects the adaptive window."""
     with Context(scheme, rows=rows, warp_density=wd, block_size=bs, batch_size=1 << 30,
-                 persist_window=window) as c:
+                 persist_window=max(0, window), persist_adaptive=window < 0) as c:
         c.seed_table()
         c.load_ops(ops)
         c.set_batches(batch_sizes)
@@ -110,7 +118,8 @@ def tune(scheme: str, rows: int, ops, batch_sizes, form: str = "persistent", log
     wd = max(WD, key=lambda w: score(w, bs, win))
     bs = max(BS, key=lambda b: score(wd, b, win))
     win = max(windows, key=lambda n: score(wd, bs, n))
-    return {"warp_density": wd, "block_size": bs, "window": win, "txps": round(score(wd, bs, win))}
+    return {"warp_density": wd, "block_size": bs, "window": max(0, win), "adaptive": win < 0,
+            "txps": round(score(wd, bs, win))}
 
 
 def main(argv=None) -> int:
@@ -122,9 +131,23 @@ def main(argv=None) -> int:
     ap.add_argument("--rows", type=int, default=10485760)
     ap.add_argument("--ntx", type=int, default=131072, help="sample transactions per bucket")
     ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--stream-batches", type=int, default=64,
+                    help="also tune bucket \"stream\" on this many batches of the batched stream (0 = skip)")
     args = ap.parse_args(argv)
     table = load_table(args.out)
-    for tb in args.thetas.split(","):
+    if args.stream_batches:
+        from . import workloads
+        full = workloads.config4(rows=args.rows)
+        nb = min(args.stream_batches, len(full.batch_sizes))
+        ntx = int(sum(full.batch_sizes[:nb]))
+        ops = type(full.ops)(full.ops.ops[: int(full.ops.tx_off[ntx])], full.ops.tx_off[: ntx + 1])
+        for scheme in args.schemes.split(","):
+            best = tune(scheme, args.rows, ops, full.batch_sizes[:nb], log=lambda s: print(s, flush=True))
+            table.setdefault(scheme, {})["stream"] = best
+            print(json.dumps({"scheme": scheme, "bucket": "stream", "best": best}), flush=True)
+            with open(args.out, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+    for tb in [x for x in args.thetas.split(",") if x]:
         for rb in args.reads.split(","):
             w = generate(args.rows, args.ntx, theta=THETAS[tb], read_ratio=READS[rb], seed=SEED)
             sizes = [min(args.batch, args.ntx - s) for s in range(0, args.ntx, args.batch)]

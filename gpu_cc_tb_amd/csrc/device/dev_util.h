@@ -124,6 +124,15 @@ __device__ __forceinline__ uint64_t group_max64(uint64_t v) {
   return v;
 }
 template <int G>
+__device__ __forceinline__ uint64_t group_min64(uint64_t v) {
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) {
+    const uint64_t o = shfl_xor64(v, m);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+template <int G>
 __device__ __forceinline__ uint64_t group_sum64(uint64_t v) {
 #pragma unroll
   for (int m = 1; m < G; m <<= 1) v += shfl_xor64(v, m);

@@ -78,15 +78,35 @@ __device__ __forceinline__ uint32_t op_index(int k) {
 // same order; an immediate retry (benchmark/ycsb.cu:99-104) can then repeat
 // forever.  The restart is delayed by a random number of units in
 // [0, 2^min(attempts, 10)), where a unit is the duration of the failed attempt
-// (at least backoff_ticks): retries spread over rounds, not just over ticks.
+// (at least backoff_ticks, at most BACKOFF_UNIT_MAX): retries spread over
+// rounds, not just over ticks.  The cap matters for attempts that waited
+// (wait-die, T/O reads of uncommitted rows): their duration includes the
+// wait, and 1024 such units would idle a transaction for seconds.
+// Urgent state (persistent form).  A transaction that has aborted
+// ExecArgs::urgent_attempts times (256 by default: a safety net, far beyond
+// the abort counts of a healthy run) adds itself to the urgent count of every
+// gate copy until it commits; while the count is non-zero no group takes a
+// new slot, so the transactions in flight drain as in a plain batch and the
+// urgent one finishes.  Without it a stream that keeps refilling never lowers
+// contention and a no-wait transaction can lose forever.
+
+__device__ __forceinline__ void urgent_add(const ExecArgs& a, uint64_t delta) {
+#pragma unroll 1
+  for (int s = 0; s < GATE_SHARDS; ++s) fetch_add(a.ctrs + CTR_GATE0 + 16 * s + 1, delta);
+}
+
+constexpr uint64_t BACKOFF_UNIT_MAX = 5000;  // 50 us of the 100 MHz wall clock
+
 template <int G>
 __device__ __forceinline__ void abort_attempt(TxCtx<G>& c, const ExecArgs& a, uint64_t now) {
   (void)now;
   const uint64_t t = now_ticks();
   const uint64_t dur = t - c.t_attempt;
   c.attempts++;
+  if (a.persist && c.attempts == a.urgent_attempts && (lane_id() & (G - 1)) == 0) urgent_add(a, 1);
   c.m_abort += dur;
-  const uint64_t unit = dur > a.backoff_ticks ? dur : a.backoff_ticks;
+  uint64_t unit = dur > a.backoff_ticks ? dur : a.backoff_ticks;
+  if (unit > BACKOFF_UNIT_MAX) unit = BACKOFF_UNIT_MAX;
   const uint64_t h = hash3(a.seed ^ TAG_BACKOFF, c.tx, c.attempts);
   const uint32_t span = 1u << (c.attempts < 10 ? c.attempts : 10);
   c.t_resume = t + (h % span) * unit;
@@ -101,7 +121,10 @@ __device__ __forceinline__ void finish_tx(TxCtx<G>& c, const ExecArgs& a, uint64
   c.m_commit += 1;
   c.m_att += c.attempts;
   if ((lane_id() & (G - 1)) == 0) {
+    if (a.persist && a.urgent_attempts && c.attempts >= a.urgent_attempts)
+      urgent_add(a, ~0ull);  // leave the urgent count
     a.obatch[c.tx] = a.batch_id;
+    if (a.tcommit) a.tcommit[c.tx] = now_ticks();
     a.okey0[c.tx] = key0;
     a.okey1[c.tx] = key1;
     a.tx_aborts[c.tx] = c.attempts;
@@ -153,10 +176,11 @@ __device__ __forceinline__ void raise_ts_floor(const ExecArgs& a, uint64_t seen)
 // ===========================================================================
 template <bool WAIT_DIE>
 struct TwoPL {
-  // lock_shared (twopl.cuh:233-267).  The first CAS guesses a free word (0),
-  // so an uncontended lock costs one atomic and no load.
-  static __device__ StepResult lock_shared(uint64_t* w, uint64_t self) {
-    uint64_t cur = 0;
+  static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
+  static constexpr bool kBulkRefill = true;
+  // lock_shared (twopl.cuh:233-267), continuing from an observed word `cur`
+  // (the round's first CAS guessed a free word and returned it).
+  static __device__ StepResult lock_shared(uint64_t* w, uint64_t self, uint64_t cur) {
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
       if (tpl::excl(cur)) {
@@ -172,8 +196,7 @@ struct TwoPL {
   }
   // lock_exclusive (twopl.cuh:269-303); a shared-held word always aborts
   // (twopl.cuh:285-286).
-  static __device__ StepResult lock_excl(uint64_t* w, uint64_t self) {
-    uint64_t cur = 0;
+  static __device__ StepResult lock_excl(uint64_t* w, uint64_t self, uint64_t cur) {
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
       if (cur == 0) {
@@ -210,12 +233,26 @@ struct TwoPL {
       }
       if (c.ph == PH_OPS) {
         bool conflict = false, pending = false;
+        // First lock attempt of every unheld op at once: a CAS that guesses
+        // a free word, all issued before any result is examined, so an
+        // uncontended transaction locks in one atomic latency.
+        uint64_t cur[K];
+        bool got[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const uint32_t m = c.meta[k];
+          cur[k] = 0;
+          got[k] = false;
+          if ((m & M_VALID) && !(m & M_HELD))
+            got[k] = cas(a.cc + c.row[k], cur[k], (m & M_WRITE) ? tpl::make_excl(c.tx) : tpl::CNT_ONE);
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const uint32_t m = c.meta[k];
           if ((m & M_VALID) && !(m & M_HELD)) {
             uint64_t* w = a.cc + c.row[k];
-            const StepResult r = (m & M_WRITE) ? lock_excl(w, c.tx) : lock_shared(w, c.tx);
+            StepResult r = R_OK;
+            if (!got[k]) r = (m & M_WRITE) ? lock_excl(w, c.tx, cur[k]) : lock_shared(w, c.tx, cur[k]);
             if (r == R_OK)
               c.meta[k] = m | M_HELD;
             else if (r == R_ABORT)
@@ -269,14 +306,12 @@ struct TwoPL {
 // the data and clears the mark; abort restores the saved words.  The
 // serialization order is timestamp order.
 // ===========================================================================
-// After a cell of the version in `cur` (observed before the cell load) was
-// read: leave rts >= ts on that same version.  Other readers raising rts
-// meanwhile leave the version, and the value read, valid, so only a change of
-// the version bits sends the caller back to re-read (false; cur then holds the
-// current word).
-__device__ __forceinline__ bool hold_version(uint64_t* w, uint64_t ts, uint64_t& cur) {
-  const uint64_t ver = tsw::version(cur);
-  bool fresh = false;  // cur was observed after the cell load
+// After a cell of version `ver` was read: leave rts >= ts on that same
+// version.  `cur` is a word observed before the cell load (fresh = false) or
+// after it (fresh = true).  Other readers raising rts meanwhile leave the
+// version, and the value read, valid, so only a change of the version bits
+// sends the caller back to re-read (false; cur then holds the current word).
+__device__ __forceinline__ bool hold_version(uint64_t* w, uint64_t ts, uint64_t ver, uint64_t& cur, bool fresh) {
 #pragma unroll 1
   for (int t = 0; t < CAS_TRIES; ++t) {
     if (fresh && tsw::version(cur) != ver) return false;
@@ -292,7 +327,64 @@ __device__ __forceinline__ bool hold_version(uint64_t* w, uint64_t ts, uint64_t&
   return false;
 }
 
+// Head reads of T/O and MVCC for every pending read of the lane's slots,
+// each step issued for all slots before its wait: the word; then the cell of
+// every slot whose head is committed and old enough (wts <= ts); then the
+// first rts raise (CAS) or re-check of each.  res[k]: R_OK (v[k] read and
+// held), R_WAIT (uncommitted head, or the version moved: retry next round) or
+// R_ABORT (head younger than ts: T/O aborts, MVCC walks the chain instead);
+// cur[k] is the last word observed.
+template <int G>
+__device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs& a, uint64_t ts,
+                                              uint64_t (&cur)[TxCtx<G>::K], uint64_t (&v)[TxCtx<G>::K],
+                                              bool (&rd)[TxCtx<G>::K], StepResult (&res)[TxCtx<G>::K]) {
+  constexpr int K = TxCtx<G>::K;
+  bool cand[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t m = c.meta[k];
+    rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
+    cur[k] = rd[k] ? ld(a.cc + c.row[k]) : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    res[k] = R_WAIT;
+    cand[k] = false;
+    if (!rd[k]) continue;
+    if (tsw::unc(cur[k])) continue;
+    if (tsw::wts(cur[k]) > ts) {
+      res[k] = R_ABORT;
+      continue;
+    }
+    cand[k] = true;
+  }
+  order_acquire();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = cand[k] ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
+  order_acquire();
+  uint64_t ver[K];
+  bool ok[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    ver[k] = tsw::version(cur[k]);
+    ok[k] = false;
+    if (!cand[k]) continue;
+    if (tsw::rts(cur[k]) >= ts)
+      cur[k] = ld(a.cc + c.row[k]);  // nothing to raise: re-check the version
+    else
+      ok[k] = cas(a.cc + c.row[k], cur[k], tsw::make(0, ts, tsw::wts(cur[k])));
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!cand[k]) continue;
+    if (!ok[k]) ok[k] = hold_version(a.cc + c.row[k], ts, ver[k], cur[k], true);
+    res[k] = ok[k] ? R_OK : R_WAIT;
+  }
+}
+
 struct TimestampOrdering {
+  static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
+  static constexpr bool kBulkRefill = true;
   static __device__ StepResult read(uint64_t* w, uint64_t* cell, uint64_t ts, uint64_t& val, uint64_t& seen) {
     uint64_t cur = ld(w);
 #pragma unroll 1
@@ -305,7 +397,7 @@ struct TimestampOrdering {
       order_acquire();
       val = ld(cell);
       order_acquire();
-      if (hold_version(w, ts, cur)) return R_OK;
+      if (hold_version(w, ts, tsw::version(cur), cur, false)) return R_OK;
     }
     return R_WAIT;
   }
@@ -347,16 +439,20 @@ struct TimestampOrdering {
     bool abort = false;
     if (c.ph == PH_OPS) {
       bool pending = false;
+      uint64_t cur[K], v[K];
+      bool rd[K];
+      StepResult r[K];
+      ts_head_reads<G>(c, a, c.ts, cur, v, rd, r);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const uint32_t m = c.meta[k];
-        if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
-        uint64_t v;
-        const StepResult r = read(a.cc + c.row[k], cell_ptr(a, c.row[k], m), c.ts, v, seen);
-        if (r == R_OK) {
-          c.meta[k] = m | M_DONE;
-          a.reads[c.op0 + op_index<G>(k)] = v;
-        } else if (r == R_ABORT) {
+        if (!rd[k]) continue;
+        if (r[k] == R_OK) {
+          c.meta[k] |= M_DONE;
+          a.reads[c.op0 + op_index<G>(k)] = v[k];
+        } else if (r[k] == R_ABORT) {
+          // a younger write committed (to.cuh:148-152)
+          const uint64_t y = tsw::wts(cur[k]);
+          seen = y > seen ? y : seen;
           abort = true;
         } else {
           pending = true;
@@ -421,6 +517,8 @@ struct TimestampOrdering {
 // [1] old value of the written column, [2] previous node + 1 | column << 32.
 // ===========================================================================
 struct Mvcc {
+  static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
+  static constexpr bool kBulkRefill = true;
   static __device__ StepResult read(const ExecArgs& a, uint32_t row, uint32_t col, uint64_t ts,
                                     uint64_t& val) {
     uint64_t* w = a.cc + row;
@@ -433,7 +531,7 @@ struct Mvcc {
         order_acquire();
         val = ld(cell);
         order_acquire();
-        if (hold_version(w, ts, cur)) return R_OK;
+        if (hold_version(w, ts, tsw::version(cur), cur, false)) return R_OK;
         continue;
       }
       // Walk back to the newest version with wts <= ts.  The head value and
@@ -496,12 +594,19 @@ struct Mvcc {
     bool commit = false;
     if (!c.done && c.ph == PH_OPS) {
       bool pending = false;
+      uint64_t cur[K], hv[K];
+      bool rd[K];
+      StepResult r[K];
+      ts_head_reads<G>(c, a, c.ts, cur, hv, rd, r);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
+        if (!rd[k]) continue;
         const uint32_t m = c.meta[k];
-        if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
-        uint64_t v;
-        if (read(a, c.row[k], m >> M_COL_SHIFT, c.ts, v) == R_OK) {
+        uint64_t v = hv[k];
+        // a head younger than ts: walk the version chain (never aborts)
+        StepResult rk = r[k];
+        if (rk == R_ABORT) rk = read(a, c.row[k], m >> M_COL_SHIFT, c.ts, v);
+        if (rk == R_OK) {
           c.meta[k] = m | M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v;
         } else {
@@ -596,6 +701,53 @@ struct Mvcc {
 };
 
 // ===========================================================================
+// OCC helpers shared by Silo and TicToc (both keep a lock in bit 0 of the
+// row word).
+// ===========================================================================
+// Consistent snapshot of every pending read of the lane's slots: the word,
+// the cell, the word again; each step is issued for all slots before its
+// wait, so a round costs three memory latencies however many slots read.
+// ok[k]: slot k read and the word was unchanged and unlocked; w1[k] the word.
+template <int G>
+__device__ __forceinline__ void occ_snapshot(const TxCtx<G>& c, const ExecArgs& a, uint64_t (&w1)[TxCtx<G>::K],
+                                             uint64_t (&v)[TxCtx<G>::K], bool (&rd)[TxCtx<G>::K],
+                                             bool (&ok)[TxCtx<G>::K]) {
+  constexpr int K = TxCtx<G>::K;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t m = c.meta[k];
+    rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
+    w1[k] = rd[k] ? ld(a.cc + c.row[k]) : 0;
+  }
+  order_acquire();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = rd[k] ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
+  order_acquire();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint64_t w2 = rd[k] ? ld(a.cc + c.row[k]) : 0;
+    ok[k] = rd[k] && w1[k] == w2 && !(w1[k] & 1u);
+  }
+}
+
+// Lock every unheld write of the lane's slots with one fetch_or each, all
+// issued before any result is examined.  old[k]: the word before the lock;
+// got[k]: the lock was free (now ours); tried[k]: slot k was attempted.
+template <int G>
+__device__ __forceinline__ void occ_lock(const TxCtx<G>& c, const ExecArgs& a, uint64_t (&old)[TxCtx<G>::K],
+                                         bool (&tried)[TxCtx<G>::K], bool (&got)[TxCtx<G>::K]) {
+  constexpr int K = TxCtx<G>::K;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t m = c.meta[k];
+    tried[k] = (m & M_VALID) && (m & M_WRITE) && !(m & M_HELD);
+    old[k] = tried[k] ? fetch_or(a.cc + c.row[k], 1ull) : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) got[k] = tried[k] && !(old[k] & 1u);
+}
+
+// ===========================================================================
 // Silo OCC (include/silo.cuh:70-339).
 // Read: consistent snapshot (word, data, word unchanged and unlocked).
 // Validation: lock the write set (no-wait), draw the commit ticket, compute
@@ -605,6 +757,8 @@ struct Mvcc {
 // is the serialization order.
 // ===========================================================================
 struct Silo {
+  static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
+  static constexpr bool kBulkRefill = true;
   template <int G>
   static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
     constexpr int K = TxCtx<G>::K;
@@ -617,23 +771,19 @@ struct Silo {
       }
       if (c.ph == PH_OPS) {
         bool pending = false;
+        uint64_t w1[K], v[K];
+        bool rd[K], ok[K];
+        occ_snapshot<G>(c, a, w1, v, rd, ok);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const uint32_t m = c.meta[k];
-          if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
-          uint64_t* w = a.cc + c.row[k];
-          const uint64_t w1 = ld(w);
-          order_acquire();
-          const uint64_t v = ld(cell_ptr(a, c.row[k], m));
-          order_acquire();
-          const uint64_t w2 = ld(w);
-          if (w1 != w2 || silo::locked(w1)) {
+          if (!rd[k]) continue;
+          if (!ok[k]) {
             pending = true;
             continue;
           }
-          c.st[k] = silo::ts(w1);
-          c.meta[k] = m | M_DONE;
-          a.reads[c.op0 + op_index<G>(k)] = v;
+          c.st[k] = silo::ts(w1[k]);
+          c.meta[k] |= M_DONE;
+          a.reads[c.op0 + op_index<G>(k)] = v[k];
         }
         if (group_all<G>(!pending))
           c.ph = PH_LOCK;
@@ -642,17 +792,18 @@ struct Silo {
       }
       if (c.ph == PH_LOCK) {
         bool conflict = false, pending = false;
+        // lock (silo.cuh:199-215) as one fetch_or: the old word says whether
+        // the lock was free (then it is ours) and carries the current TID;
+        // setting an already-set bit changes nothing, so no undo is needed
+        uint64_t old[K];
+        bool tried[K], got[K];
+        occ_lock<G>(c, a, old, tried, got);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const uint32_t m = c.meta[k];
-          if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_HELD)) continue;
-          // lock (silo.cuh:199-215) as one fetch_or: the old word says whether
-          // the lock was free (then it is ours) and carries the current TID;
-          // setting an already-set bit changes nothing, so no undo is needed
-          const uint64_t cur = fetch_or(a.cc + c.row[k], 1ull);
-          if (!silo::locked(cur)) {
-            c.st[k] = silo::ts(cur);
-            c.meta[k] = m | M_HELD;
+          if (!tried[k]) continue;
+          if (got[k]) {
+            c.st[k] = silo::ts(old[k]);
+            c.meta[k] |= M_HELD;
           } else {
             conflict = true;
           }
@@ -716,6 +867,8 @@ struct Silo {
 // is the serialization order.
 // ===========================================================================
 struct Tictoc {
+  static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
+  static constexpr bool kBulkRefill = true;
   template <int G>
   static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
     constexpr int K = TxCtx<G>::K;
@@ -729,23 +882,19 @@ struct Tictoc {
       }
       if (c.ph == PH_OPS) {
         bool pending = false;
+        uint64_t w1[K], v[K];
+        bool rd[K], ok[K];
+        occ_snapshot<G>(c, a, w1, v, rd, ok);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const uint32_t m = c.meta[k];
-          if (!(m & M_VALID) || (m & (M_DONE | M_WRITE))) continue;
-          uint64_t* w = a.cc + c.row[k];
-          const uint64_t w1 = ld(w);
-          order_acquire();
-          const uint64_t v = ld(cell_ptr(a, c.row[k], m));
-          order_acquire();
-          const uint64_t w2 = ld(w);
-          if (w1 != w2 || tictoc::locked(w1)) {
+          if (!rd[k]) continue;
+          if (!ok[k]) {
             pending = true;
             continue;
           }
-          c.st[k] = w1;
-          c.meta[k] = m | M_DONE;
-          a.reads[c.op0 + op_index<G>(k)] = v;
+          c.st[k] = w1[k];
+          c.meta[k] |= M_DONE;
+          a.reads[c.op0 + op_index<G>(k)] = v[k];
         }
         if (group_all<G>(!pending))
           c.ph = PH_LOCK;
@@ -754,17 +903,18 @@ struct Tictoc {
       }
       if (c.ph == PH_LOCK) {
         bool conflict = false, pending = false;
+        // lock (tictoc.cuh:198-215) as one fetch_or.  A reader's rts
+        // extension never CASes a locked word (tictoc.cuh:316-338), so the
+        // old word stays valid until this attempt unlocks or writes.
+        uint64_t old[K];
+        bool tried[K], got[K];
+        occ_lock<G>(c, a, old, tried, got);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const uint32_t m = c.meta[k];
-          if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_HELD)) continue;
-          // lock (tictoc.cuh:198-215) as one fetch_or.  A reader's rts
-          // extension never CASes a locked word (tictoc.cuh:316-338), so the
-          // old word stays valid until this attempt unlocks or writes.
-          const uint64_t cur = fetch_or(a.cc + c.row[k], 1ull);
-          if (!tictoc::locked(cur)) {
-            c.st[k] = cur;
-            c.meta[k] = m | M_HELD;
+          if (!tried[k]) continue;
+          if (got[k]) {
+            c.st[k] = old[k];
+            c.meta[k] |= M_HELD;
           } else {
             conflict = true;
           }
@@ -854,42 +1004,79 @@ struct Ordered {
     constexpr int K = TxCtx<G>::K;
     if (c.done) return;
     const uint64_t now = now_ticks();
-    if (c.ph == PH_START) {
+    if (c.ph == PH_START) {  // c.st[k] holds each op's turn (load_tx)
       begin_attempt(c, now);
       c.ph = PH_OPS;
-      // each op's turn on its row, kept in registers for the polls
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        if (c.meta[k] & M_VALID) c.st[k] = a.need[c.op0 + op_index<G>(k)];
     }
-    bool pending = false;
+    if (now < c.t_resume) return;  // resting: no op can have its turn yet
+    // Poll every waiting op's row counter at once (independent loads, one
+    // memory latency for all slots), then run the ops whose turn has come:
+    // one acquire before their cell accesses, one release before the counter
+    // increments that pass the turn on.
+    uint32_t seen[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
-      if (!(m & M_VALID) || (m & M_DONE)) continue;
-      const uint32_t j = op_index<G>(k);
-      uint32_t* ctr = a.counters + c.row[k];
-      if (ld32(ctr) < (uint32_t)c.st[k]) {
-        pending = true;
-        continue;
+      seen[k] = ((m & M_VALID) && !(m & M_DONE)) ? ld32(a.counters + c.row[k]) : 0u;
+    }
+    bool pending = false, any = false;
+    bool ready[K];
+    uint32_t gap = 0xffffffffu;  // fewest turns any still-waiting op is away from its own
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t m = c.meta[k];
+      const bool waiting = (m & M_VALID) && !(m & M_DONE);
+      ready[k] = waiting && seen[k] >= (uint32_t)c.st[k];
+      pending |= waiting && !ready[k];
+      any |= ready[k];
+      if (waiting && !ready[k]) {
+        const uint32_t d = (uint32_t)c.st[k] - seen[k];
+        gap = d < gap ? d : gap;
       }
+    }
+    if (any) {
       order_acquire();
-      uint64_t* cell = cell_ptr(a, c.row[k], m);
-      if (m & M_WRITE)
-        st(cell, write_value(a.seed, c.tx, j));
-      else
-        a.reads[c.op0 + j] = ld(cell);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (!ready[k]) continue;
+        const uint32_t m = c.meta[k];
+        const uint32_t j = op_index<G>(k);
+        uint64_t* cell = cell_ptr(a, c.row[k], m);
+        if (m & M_WRITE)
+          st(cell, write_value(a.seed, c.tx, j));
+        else
+          a.reads[c.op0 + j] = ld(cell);
+      }
       order_release();
-      fetch_add32(ctr, 1u);
-      c.meta[k] = m | M_DONE;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (!ready[k]) continue;
+        fetch_add32(a.counters + c.row[k], 1u);
+        c.meta[k] |= M_DONE;
+      }
     }
     if (group_all<G>(!pending)) {
       const uint64_t key0 = a.tx_rank ? a.tx_rank[c.tx] : c.tx;
       finish_tx(c, a, now, key0, c.tx);
     } else {
       c.m_wait += now - c.t_round;
+      // An op `gap` turns short of its own cannot run before `gap` handoffs
+      // on its row: rest until then (POLL_TICKS_PER_TURN each, at most
+      // POLL_REST_MAX) instead of polling every round; deep waiters on a hot
+      // row then stop flooding the memory system the row's chain depends on.
+      const uint64_t turns = group_min64<G>(gap);
+      if (turns > 1) {
+        const uint64_t rest = (turns - 1) * POLL_TICKS_PER_TURN;
+        c.t_resume = now + (rest < POLL_REST_MAX ? rest : POLL_REST_MAX);
+      }
     }
   }
+  static constexpr bool kRestInOps = true;  // t_resume also paces the polls of PH_OPS
+  // Refill at once: a done group that waited for its wave would idle behind
+  // groups queued on hot rows (measured: 29% slower on the batched stream).
+  static constexpr bool kBulkRefill = false;
+  static constexpr uint64_t POLL_TICKS_PER_TURN = 50;  // 0.5 us per handoff on the row
+  static constexpr uint64_t POLL_REST_MAX = 12800;     // 128 us
 };
 
 // ===========================================================================
@@ -933,6 +1120,7 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
       if (row < a.rows) {
         c.row[k] = row;
         c.meta[k] = M_VALID | (op_is_write(op) ? M_WRITE : 0u) | (op_col(op) << M_COL_SHIFT);
+        if (a.need) c.st[k] = a.need[c.op0 + j];  // ordered schemes: the op's turn on its row
       } else {
         c.m_miss++;
       }
@@ -1000,8 +1188,9 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
 // ---------------------------------------------------------------------------
 constexpr uint64_t FLUSH_TICKS = 10000;  // 100 us of the 100 MHz wall clock
 constexpr uint64_t CTL_TICKS = 20000;    // 200 us
-constexpr uint64_t WIN_TICKS = 2000;     // a wave re-reads the window at most every 20 us
-constexpr uint64_t PARK_TICKS = 10000;   // parked groups re-check the stream every 100 us
+constexpr uint64_t GATE_TICKS = 5000;    // a refilling wave re-reads its gate copy at most every 50 us
+constexpr uint64_t PARK_TICKS = 20000;   // a wave with only parked groups: every 200 us
+constexpr uint64_t STALL_TICKS = 100000; // parked waves sample the slot counter every 1 ms
 constexpr uint64_t WIN_MIN = 256;
 constexpr uint32_t CTL_BLOCK_STRIDE = 64;  // wave 0 of every 64th block may run the controller
 
@@ -1019,13 +1208,14 @@ __device__ __forceinline__ void adapt_window(const ExecArgs& a, uint64_t now, ui
   if (dc + da < 256) return;  // too few events to judge
   st(ctl + 1, cm);
   st(ctl + 2, ab);
-  uint64_t w = ld(a.ctrs + CTR_WIN);
+  uint64_t w = ld(a.ctrs + CTR_GATE0);
   if (w > wmax) w = wmax;
   if (da > 2 * dc)
     w = w - w / 4 < WIN_MIN ? WIN_MIN : w - w / 4;
   else if (2 * da < dc)
     w = w + w / 2 + 64 > wmax ? wmax : w + w / 2 + 64;
-  st(a.ctrs + CTR_WIN, w);
+#pragma unroll 1
+  for (int s = 0; s < GATE_SHARDS; ++s) st(a.ctrs + CTR_GATE0 + 16 * s, w);
 }
 
 template <class S, int G>
@@ -1036,11 +1226,12 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups)
   const bool adapt = a.adapt != 0;
   const bool controller = adapt && threadIdx.x < 64 && (blockIdx.x % CTL_BLOCK_STRIDE) == 0;
   uint64_t* slot_ctr = a.ctrs + CTR_SLOT;
+  uint64_t* gate = a.ctrs + CTR_GATE0 + 16 * (blockIdx.x & (GATE_SHARDS - 1));
   bool finished = false;
   uint64_t acc_c = 0, acc_a = 0;
   uint64_t now = now_ticks();
-  uint64_t next_flush = now + FLUSH_TICKS, next_ctl = now, next_park = now, next_w = now;
-  uint64_t wcache = adapt ? 0 : ~0ull;
+  uint64_t next_flush = now + FLUSH_TICKS, next_ctl = now, next_gate = now, next_stall = now;
+  uint64_t wcache = ~0ull, urgent = 0, handed_out = 0;
   uint64_t seen_slot = ~0ull, seen_at = now;
   c.live = false;
   c.done = true;
@@ -1050,28 +1241,50 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups)
     // admission (the wall clock is a scalar read: `now` is wave-uniform)
     const bool need = c.done && !finished;
     bool stalled = false;
-    if (ballot(need) != 0) {
-      if (adapt && now >= next_w) {
-        wcache = ld(a.ctrs + CTR_WIN);
-        next_w = now + WIN_TICKS;
+    // Bulk refill (schemes with kBulkRefill): the groups of a wave that are
+    // done wait until half of the wave's groups are (or none is live), then
+    // refill together, so the wave pays the refill's chain of dependent loads
+    // (slot, offsets, ops and rows) once for many transactions, like a freshly
+    // dispatched wave of the plain form; refilling one group at a time puts
+    // that chain into most rounds (measured: Silo 19% faster on the batched
+    // stream).
+    const uint32_t ngw = (uint32_t)__popcll(ballot(leader));
+    const uint32_t nneed = (uint32_t)__popcll(ballot(need && leader));
+    const bool bulk = !S::kBulkRefill || 2 * nneed >= ngw || ballot(!c.done) == 0;
+    if (bulk && ballot(need) != 0) {
+      if (now >= next_gate) {
+        // this block's gate copy: window, urgent count, stream handed out
+        const uint64_t gw = ld(gate), gu = ld(gate + 1), gd = ld(gate + 2);
+        wcache = adapt ? gw : ~0ull;
+        urgent = gu;
+        handed_out = gd;
+        next_gate = now + (ballot(!c.done) != 0 ? GATE_TICKS : PARK_TICKS);
       }
-      const bool take = need && grank < wcache;
+      if (handed_out && need) finished = true;
+      const bool take = need && !finished && urgent == 0 && grank < wcache;
       if (ballot(take) != 0) {
         const uint64_t got = group_bcast64<G>(wave_fetch_inc(slot_ctr, take && leader));
         if (take) {
-          if (got < nslots)
+          if (got < nslots) {
             load_tx(c, a, a.slot_lo + got, now_ticks());
-          else
+          } else {
             finished = true;
+            // the group that drew the first slot past the end tells every gate
+            if (got == nslots && leader) {
+#pragma unroll 1
+              for (int s = 0; s < GATE_SHARDS; ++s) st(a.ctrs + CTR_GATE0 + 16 * s + 2, 1);
+            }
+          }
         }
       }
-      if (ballot(need && !take) != 0 && now >= next_park) {
-        // parked groups: finished once the stream is handed out; a counter
-        // that has not moved for the watchdog period means no group progresses
-        next_park = now + PARK_TICKS;
+      const bool parked = need && !finished && !take;
+      if (ballot(parked) != 0 && now >= next_stall) {
+        // parked groups: a slot counter that has not moved for the watchdog
+        // period means that no group progresses
+        next_stall = now + STALL_TICKS;
         const uint64_t cur = ld(slot_ctr);
         if (cur >= nslots) {
-          if (need && !take) finished = true;
+          if (parked) finished = true;
         } else if (cur != seen_slot) {
           seen_slot = cur;
           seen_at = now;
@@ -1089,8 +1302,8 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups)
     const bool done0 = c.done;
     if (ballot(!c.done) == 0)
       __builtin_amdgcn_s_sleep(64);  // every group of the wave is parked
-    else if (ballot(!c.done && !(c.ph == PH_START && c.t_round < c.t_resume)) == 0)
-      __builtin_amdgcn_s_sleep(4);
+    else if (ballot(!c.done && !((c.ph == PH_START || S::kRestInOps) && c.t_round < c.t_resume)) == 0)
+      __builtin_amdgcn_s_sleep(4);  // every live group is resting
     else
       S::round(c, a);
     c.t_round = now = now_ticks();
@@ -1135,8 +1348,8 @@ __global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
 #pragma unroll 1
     while (true) {
       if (ballot(!c.done) == 0) break;
-      // every live group of the wave is backing off: sleep instead of spinning
-      if (ballot(!c.done && !(c.ph == PH_START && c.t_round < c.t_resume)) == 0)
+      // every live group of the wave is resting: sleep instead of spinning
+      if (ballot(!c.done && !((c.ph == PH_START || S::kRestInOps) && c.t_round < c.t_resume)) == 0)
         __builtin_amdgcn_s_sleep(4);
       else
         S::round(c, a);

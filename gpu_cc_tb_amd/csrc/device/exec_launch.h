@@ -37,9 +37,11 @@ hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream
 }
 
 template <class S>
-hipError_t launch_exec_g(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s) {
+hipError_t launch_exec_g(int G, const ExecArgs& args, uint64_t nslots, int threads, bool persist, hipStream_t s) {
   if (nslots == 0) return hipSuccess;
   if (threads <= 0 || threads > 1024) return hipErrorInvalidValue;
+  ExecArgs a = args;
+  a.persist = persist ? 1u : 0u;
 #define GC_CASE(g)                                                                  \
   case g:                                                                           \
     return persist ? launch_one<S, g, true>(a, nslots, threads, s)                  \

@@ -458,7 +458,6 @@ __global__ void k_adapt_begin(uint64_t* ctrs, uint64_t window) {
   const int i = threadIdx.x;
   if (i == 0) {
     dev::st(ctrs + CTR_SLOT, 0);
-    dev::st(ctrs + CTR_WIN, window);
     dev::st(ctrs + CTR_CTL, 0);
     dev::st(ctrs + CTR_CTL + 1, 0);
     dev::st(ctrs + CTR_CTL + 2, 0);
@@ -466,6 +465,11 @@ __global__ void k_adapt_begin(uint64_t* ctrs, uint64_t window) {
   if (i < STAT_SHARDS) {
     dev::st(ctrs + CTR_STAT0 + 16 * i, 0);
     dev::st(ctrs + CTR_STAT0 + 16 * i + 1, 0);
+  }
+  if (i < GATE_SHARDS) {
+    dev::st(ctrs + CTR_GATE0 + 16 * i, window);
+    dev::st(ctrs + CTR_GATE0 + 16 * i + 1, 0);
+    dev::st(ctrs + CTR_GATE0 + 16 * i + 2, 0);
   }
 }
 

@@ -164,14 +164,18 @@ constexpr int CTR_POOL = 16;
 constexpr int CTR_TS0 = 32;
 constexpr int TS_SHARDS = 16;
 constexpr int TS_SHARD_BITS = 4;
-// Adaptive admission of the persistent form: next stream slot, admission
-// window, controller state, and per-shard commit / abort counts.
+// Admission state of the persistent form: next stream slot, controller
+// state, per-shard commit / abort counts, and GATE_SHARDS copies of the gate
+// a wave reads before taking slots (one copy per block index mod GATE_SHARDS,
+// each on its own line, so the polls of thousands of waves spread out):
+// [0] admission window, [1] transactions in urgent state, [2] stream handed out.
 constexpr int CTR_SLOT = CTR_TS0 + 16 * TS_SHARDS;
-constexpr int CTR_WIN = CTR_SLOT + 16;
-constexpr int CTR_CTL = CTR_WIN + 16;   // [0] next update tick, [1] commits, [2] aborts at the last update
+constexpr int CTR_CTL = CTR_SLOT + 16;   // [0] next update tick, [1] commits, [2] aborts at the last update
 constexpr int CTR_STAT0 = CTR_CTL + 16;  // shard s: [16*s] commits, [16*s + 1] aborted attempts
 constexpr int STAT_SHARDS = 16;
-constexpr int CTR_WORDS = CTR_STAT0 + 16 * STAT_SHARDS;
+constexpr int CTR_GATE0 = CTR_STAT0 + 16 * STAT_SHARDS;
+constexpr int GATE_SHARDS = 16;
+constexpr int CTR_WORDS = CTR_GATE0 + 16 * GATE_SHARDS;
 // Waves add their totals into one of METRIC_SLOTS 128-byte slots (by block
 // index) so that the end-of-kernel atomics do not pile onto one address; the
 // host sums the slots.
@@ -240,7 +244,10 @@ struct ExecArgs {
   uint64_t watchdog_ticks; // a wave gives up after this many ticks (0 = never)
   uint64_t* dbg;           // optional: 4 u64 per op, written for transactions the watchdog stopped
   uint64_t persist_groups; // persistent form: at most this many transactions in flight (0 = device-filling)
-  uint32_t adapt;          // persistent form: adapt the window to the abort rate (slots from CTR_SLOT)
+  uint32_t adapt;          // persistent form: adapt the window to the abort rate
+  uint32_t persist;        // 1 in persistent launches (slots from CTR_SLOT, gates, urgent state)
+  uint32_t urgent_attempts; // persistent form: aborts that make a transaction urgent (0 = never)
+  uint64_t* tcommit;       // optional, per tx: wall-clock tick of its commit (serving form)
 };
 
 }  // namespace gcctb

@@ -83,6 +83,7 @@ struct gcctb_ctx {
   int G = 1;
   int threads = 1024;
   uint64_t watchdog_ticks = 0;
+  uint32_t urgent_attempts = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   // table + index
@@ -122,6 +123,8 @@ struct gcctb_ctx {
   uint64_t nranks = 0;
   uint64_t last_committed = 0;
   bool rows_probed = false;  // rows_out holds every op's row (persistent form)
+  gcctb::DevBuf<uint64_t> tcommit;  // serving form: per-tx commit tick
+  bool record_commit = false;
   std::vector<uint32_t> h_rank_counts;
   std::vector<uint64_t> custom_batches;  // uneven batch sizes (empty: batch_size)
   // hipGraph of the per-batch sequence (graph form), rebuilt when inputs change
@@ -201,6 +204,7 @@ class SchemeHost {
     a.ctrs = ctx.ctrs.p;
     a.rows_out = ctx.rows_out.p;
     a.rows_in = ctx.rows_probed ? ctx.rows_out.p : nullptr;
+    a.tcommit = ctx.record_commit ? ctx.tcommit.p : nullptr;
     a.reads = ctx.reads.p;
     a.obatch = ctx.obatch.p;
     a.okey0 = ctx.okey0.p;
@@ -212,6 +216,7 @@ class SchemeHost {
     a.backoff_ticks = ctx.cfg.backoff_ticks;
     a.tx_rank = nullptr;
     a.watchdog_ticks = ctx.watchdog_ticks;
+    a.urgent_attempts = ctx.urgent_attempts;
     a.dbg = ctx.dbg.p;
     a.persist_groups = ctx.cfg.persist_window;
     a.adapt = ctx.cfg.persist_adaptive ? 1u : 0u;
@@ -473,6 +478,9 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
     const char* wd = std::getenv("GCCTB_WATCHDOG_S");
     const double sec = wd ? std::atof(wd) : 20.0;
     c->watchdog_ticks = sec > 0 ? (uint64_t)(sec * 1e8) : 0;
+    // persistent form: aborts after which a transaction pauses admission
+    // until it commits (GCCTB_URGENT_ATTEMPTS, 0 = never)
+    if (const char* ua = std::getenv("GCCTB_URGENT_ATTEMPTS")) c->urgent_attempts = (uint32_t)std::atoi(ua);
     const char* dump = std::getenv("GCCTB_DEBUG_DUMP");
     if (dump && *dump) c->dbg_path = dump;
   }
@@ -975,6 +983,13 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
     HIP_TRY(hipStreamSynchronize(s));
     if (c->scheme == SCHEME_MVCC) c->pool_cap = std::min(c->pool_cap, pool_nodes);
     std::vector<double> lat(c->ntx, 0.0);
+    c->tcommit.alloc(c->ntx);
+    c->record_commit = true;
+    struct LaunchRec {
+      uint64_t lo, hi, b0, b1;
+      double done_ms;
+    };
+    std::vector<LaunchRec> recs;
     double busy = 0, launch_txs = 0;
     const auto t0 = clk::now();
     auto since = [&](clk::time_point t) { return std::chrono::duration<double, std::milli>(t - t0).count(); };
@@ -1015,16 +1030,35 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
       const auto td = clk::now();
       const double done = since(td);
       busy += std::chrono::duration<double, std::milli>(td - tl).count();
-      for (uint64_t b = next; b < end; ++b) {
-        const double arr = arrival_ms ? arrival_ms[b] : 0.0;
-        for (uint64_t t = bfirst[b]; t < bfirst[b + 1]; ++t) lat[t] = done - arr;
-      }
+      recs.push_back({lo, hi, next, end, done});
       launch_txs += (double)txs;
       ++launches;
       next = end;
     }
     st.tot_ms = since(clk::now());
+    c->record_commit = false;
     c->pool_cap = saved_pool_cap;
+    // Commit latency.  The kernels stamp each commit with the device wall
+    // clock; a launch's last commit happened no later than the host saw the
+    // launch complete, so ticks map to host time through that completion
+    // (the mapping can only make a commit look later, by the restore kernel
+    // and the completion notice).
+    {
+      std::vector<uint64_t> tick(c->ntx);
+      if (c->ntx) HIP_TRY(hipMemcpy(tick.data(), c->tcommit.p, c->ntx * 8, hipMemcpyDeviceToHost));
+      int khz = 0;
+      HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+      const double ticks_per_ms = khz > 0 ? (double)khz : 1e5;
+      for (const LaunchRec& r : recs) {
+        uint64_t last = 0;
+        for (uint64_t t = r.lo; t < r.hi; ++t) last = std::max(last, tick[t]);
+        for (uint64_t b = r.b0; b < r.b1; ++b) {
+          const double arr = arrival_ms ? arrival_ms[b] : 0.0;
+          for (uint64_t t = bfirst[b]; t < bfirst[b + 1]; ++t)
+            lat[t] = r.done_ms - (double)(last - tick[t]) / ticks_per_ms - arr;
+        }
+      }
+    }
     const Metrics m = collect_metrics(c);
     st.committed = m.committed;
     st.aborts = m.abort;
@@ -1046,6 +1080,7 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
     c->last_committed = m.committed;
   } catch (const HipFailure& f) {
     c->rows_probed = false;
+    c->record_commit = false;
     c->pool_cap = saved_pool_cap;
     return hip_fail(f);
   }

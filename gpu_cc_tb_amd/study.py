@@ -51,6 +51,9 @@ def sweep(args) -> None:
                         c.seed_table()
                         c.load_ops(w)
                         st = c.run("plain", check=False)
+                    if st["tot_ms"] <= 0:
+                        print(json.dumps({"study": "sweep", "scheme": scheme, "rc": st["rc"]}), flush=True)
+                        continue
                     print(json.dumps({"study": "sweep", "scheme": scheme, "theta": th, "read_ratio": rr, "batch": b,
                                       "wd": shape["warp_density"], "bs": shape["block_size"],
                                       "txps": round(w.ntx / (st["tot_ms"] / 1e3)), "aborts": st["aborts"],
@@ -65,17 +68,23 @@ def stream(args) -> None:
                       "gen_s": round(time.time() - t0, 1)}), flush=True)
     forms = args.forms.split(",")
     for scheme in args.schemes.split(","):
-        shape = autotune.lookup(scheme, 0.5, 0.9)
+        shape = autotune.lookup(scheme, 0.5, 0.9, key="stream")
         res = {}
         for f in forms:
             parts = f.split(":")
-            form, win, adapt = parts[0], int(parts[1]) if len(parts) > 1 else 0, parts[2:] == ["a"]
+            if parts[1:] == ["tuned"]:
+                form, win, adapt = parts[0], shape["window"], shape["adaptive"]
+            else:
+                form, win, adapt = parts[0], int(parts[1]) if len(parts) > 1 else 0, parts[2:] == ["a"]
             with Context(scheme, rows=args.rows, warp_density=shape["warp_density"], block_size=shape["block_size"],
                          batch_size=1 << 30, persist_window=win, persist_adaptive=adapt) as c:
                 c.seed_table()
                 c.load_ops(s.ops)
                 c.set_batches(s.batch_sizes)
                 st = c.run(form, check=False)
+            if st["rc"] != 0 and st["tot_ms"] <= 0:
+                print(json.dumps({"study": "stream", "scheme": scheme, "form": f, "rc": st["rc"]}), flush=True)
+                continue
             res[f] = s.ntx / (st["tot_ms"] / 1e3)
             print(json.dumps({"study": "stream", "scheme": scheme, "form": f, "txps": round(res[f]),
                               "tot_ms": round(st["tot_ms"], 2), "pre_ms": round(st["pre_ms"], 2),
@@ -92,7 +101,7 @@ def serve(args) -> None:
     for scheme in args.schemes.split(","):
         shape = autotune.lookup(scheme, 0.9, 0.8)
         with Context(scheme, rows=args.rows, warp_density=shape["warp_density"], block_size=shape["block_size"],
-                     batch_size=1 << 30, persist_window=shape["window"]) as c:
+                     batch_size=1 << 30, persist_window=shape["window"], persist_adaptive=shape["adaptive"]) as c:
             c.seed_table()
             c.load_ops(s.ops)
             c.serve(s.batch_sizes, max_launch_txs=args.max_launch)
@@ -127,7 +136,9 @@ def main(argv=None) -> int:
     b.add_argument("--rows", type=int, default=10485760)
     b.add_argument("--batches", type=int, default=256)
     b.add_argument("--schemes", default=",".join(SCHEMES))
-    b.add_argument("--forms", default="plain,graph,persistent:0,persistent:4096,persistent:0:a")
+    b.add_argument("--forms", default="plain,graph,persistent:tuned,persistent:0,persistent:4096,persistent:1024,"
+                                      "persistent:0:a",
+                   help="form[:window[:a]]; persistent:tuned takes the autotuned stream entry")
     b.set_defaults(fn=stream)
     c = sub.add_parser("serve")
     c.add_argument("--rows", type=int, default=67108864)

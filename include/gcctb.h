@@ -129,8 +129,10 @@ int gcctb_run_persistent(gcctb_ctx* ctx, gcctb_stats* out);
  * peak-load case).  Table, index and CC state stay on the device; whenever it
  * is idle, every arrived batch that fits the admission limits is copied from
  * pinned host memory and run as one persistent launch.  tx_latency_ms (may be
- * null) receives, per transaction, completion of its launch minus arrival of
- * its batch.  Order keys from gcctb_fetch are {launch, key0, key1}.  Not
+ * null) receives, per transaction, its commit time minus the arrival of its
+ * batch (commits are stamped with the device wall clock and mapped to host
+ * time through the launch's completion).  Order keys from gcctb_fetch are
+ * {launch, key0, key1}.  Not
  * available for gputx (it ranks its whole input before running). */
 typedef struct gcctb_serve_config {
   uint64_t max_launch_txs;  /* admission: transactions per launch (0 = no limit) */

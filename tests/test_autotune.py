@@ -10,14 +10,17 @@ def test_buckets():
 
 
 def test_lookup_default_and_nearest():
-    assert at.lookup("silo", 0.9, 0.9, table={}) == {"warp_density": 3, "block_size": 8, "window": 0}
+    assert at.lookup("silo", 0.9, 0.9, table={}) == {"warp_density": 3, "block_size": 8, "window": 0,
+                                                      "adaptive": False}
     table = {"silo": {"t0-rr": {"warp_density": 4, "block_size": 16, "window": 0, "txps": 1},
                       "t4-rr": {"warp_density": 3, "block_size": 8, "window": 1024, "txps": 1},
+                      "t2-ro": {"warp_density": 3, "block_size": 8, "window": 0, "adaptive": True, "txps": 1},
                       "t4-rw": {"warp_density": 2, "block_size": 4, "window": 4096, "txps": 1}}}
     assert at.lookup("silo", 0.99, 0.9, table)["window"] == 1024
     assert at.lookup("silo", 0.99, 0.5, table)["window"] == 4096
     # no t3 entry: nearest skew with the same read bucket
     assert at.lookup("silo", 0.9, 0.9, table)["window"] == 1024
     assert at.lookup("silo", 0.05, 0.9, table)["block_size"] == 16
+    assert at.lookup("silo", 0.8, 1.0, table)["adaptive"] is True
     # another scheme falls back to its default shape
     assert at.lookup("gacco", 0.99, 0.9, table)["warp_density"] == 2

@@ -117,3 +117,23 @@ def test_graph_launch_cap():
         res = ctx.fetch()
     assert st["committed"] == ntx and st["batches"] == nranks
     check_replay(w, res, rows, g.SEED)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["tpl_nw", "silo", "to"])
+def test_persistent_urgent_drain(scheme, monkeypatch):
+    """A hot stream in the device-filling persistent form with the urgent
+    threshold forced down to 16 aborts: transactions that reach it pause
+    admission until they commit; the run completes and replays exactly."""
+    rows, ntx = 64, 20000
+    w = g.generate(rows, ntx, theta=0.99, read_ratio=0.5, len_min=8, len_max=8)
+    print(f"[urgent] scheme={scheme} ntx={ntx}", flush=True)
+    monkeypatch.setenv("GCCTB_URGENT_ATTEMPTS", "16")
+    with g.Context(scheme, rows=rows, warp_density=3, block_size=8, batch_size=ntx) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run("persistent")
+        res = ctx.fetch(aborts=True)
+    assert st["committed"] == ntx and st["error_flags"] == 0
+    assert int(res["aborts"].max()) >= 16, "the stream should drive some transaction into the urgent state"
+    check_replay(w, res, rows, g.SEED)
