@@ -34,7 +34,7 @@ THETAS = {"0": 0.1, "1": 0.6, "2": 0.8, "3": 0.9, "4": 0.99}
 READS = {"w": 0.5, "r": 0.9, "o": 1.0}
 
 WD = [2, 3, 4]
-BS = [4, 8, 16]
+BS = [4, 8]
 # admission windows (transactions in flight; 0 = device-filling, -1 = adaptive)
 WINDOWS = [0, 4096, 1024, 256, -1]
 
