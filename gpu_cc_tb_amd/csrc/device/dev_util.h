@@ -52,21 +52,20 @@ __device__ __forceinline__ uint64_t fetch_or(uint64_t* p, uint64_t v) {
   return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ---- order-only fences ------------------------------------------------------
-// s_waitcnt simm16 on gfx9: vmcnt[3:0]+[15:14], expcnt[6:4], lgkmcnt[11:8].
-constexpr int WAIT_VM0 = 0 | (0x7 << 4) | (0xf << 8);
-// All earlier coherent accesses of this lane have completed before any later
-// one is issued (release side: data before the flag that publishes it).
-__device__ __forceinline__ void order_release() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(WAIT_VM0);
-}
-// Later coherent accesses are issued only after the earlier ones returned
-// (acquire side: flag before the data it guards).
-__device__ __forceinline__ void order_acquire() {
-  __builtin_amdgcn_s_waitcnt(WAIT_VM0);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
+// ---- ordering between different locations ----------------------------------
+// Agent-scope release / acquire fences.  The coherent accesses above keep each
+// location coherent across the XCDs, but a store acknowledged by this XCD's
+// L2 is not yet performed at memory, and another location may still be served
+// from a line cached earlier: publishing data through a second location (cell
+// then version word, node then chain head, cell then turn counter) needs the
+// L2 write-back of the release fence (buffer_wbl2 sc1 + wait) before the flag,
+// and the reader needs the invalidate of the acquire fence (buffer_inv sc1)
+// between the flag and the data.  A wait for the earlier access alone left a
+// reader able to see a new version word with the old cell behind it.
+// release side: data before the flag that publishes it
+__device__ __forceinline__ void order_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+// acquire side: flag before the data it guards
+__device__ __forceinline__ void order_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // ---- wave / group intrinsics ----------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

@@ -553,7 +553,9 @@ struct Mvcc {
       bool broken = false;
 #pragma unroll 1
       while (cur_wts > ts) {
-        if (p == 0) {
+        // a link read from shared memory is an index: never follow one that
+        // leaves the pool (0 ends the chain; nodes are 1..pool_cap)
+        if (p == 0 || p > a.pool_cap) {
           broken = true;
           break;
         }
@@ -670,28 +672,39 @@ struct Mvcc {
     if (node + nw > a.pool_cap) {
       atomicOr((unsigned long long*)&a.metrics->error_flags, ERR_POOL_OVERFLOW);
     }
+    // Install, in three steps for all of the lane's writes at once (this
+    // attempt holds the uncommitted mark of every row it installs on):
+    // version nodes and new cells; then, after a release, the chain heads;
+    // then, after another release, the committed words.  A reader that sees
+    // a committed word therefore sees its chain head and every node below it.
+    uint64_t oldv[K];
+    uint32_t prev[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const bool h = (c.meta[k] & M_HELD) != 0;
+      oldv[k] = h ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
+      prev[k] = h ? ld32(a.vprev + c.row[k]) : 0;
+    }
+    uint64_t nd_of[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
+      nd_of[k] = ~0ull;
       if (!(m & M_HELD)) continue;
-      const uint32_t row = c.row[k];
-      const uint32_t col = m >> M_COL_SHIFT;
-      uint64_t* cell = a.cells + (uint64_t)row * a.cols + col;
       if (node < a.pool_cap) {
         uint64_t* nd = a.pool + node * 3;
-        const uint64_t old = ld(cell);
-        const uint32_t prev = ld32(a.vprev + row);
         st(nd + 0, c.st[k]);
-        st(nd + 1, old);
-        st(nd + 2, (uint64_t)prev | ((uint64_t)col << 32));
-        st(cell, write_value(a.seed, c.tx, op_index<G>(k)));
-        order_release();
-        st32(a.vprev + row, (uint32_t)(node + 1));
-      } else {
-        st(cell, write_value(a.seed, c.tx, op_index<G>(k)));
+        st(nd + 1, oldv[k]);
+        st(nd + 2, (uint64_t)prev[k] | ((uint64_t)(m >> M_COL_SHIFT) << 32));
+        nd_of[k] = node;
       }
+      st(cell_ptr(a, c.row[k], m), write_value(a.seed, c.tx, op_index<G>(k)));
       node++;
     }
+    order_release();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (nd_of[k] != ~0ull) st32(a.vprev + c.row[k], (uint32_t)(nd_of[k] + 1));
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -818,7 +831,7 @@ struct Silo {
         }
       }
     }
-    order_acquire();
+    // the ticket follows the locks: their fetch_or results were consumed
     uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, validate && (lane_id() & (G - 1)) == 0);
     if (!validate) return;
     ticket = group_bcast64<G>(ticket);
@@ -970,7 +983,7 @@ struct Tictoc {
         }
       }
     }
-    order_acquire();
+    // the ticket follows validation: its loads and CAS results were consumed
     uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, commit && (lane_id() & (G - 1)) == 0);
     if (!commit) return;
     ticket = group_bcast64<G>(ticket);

@@ -463,6 +463,12 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   if (cfg->batch_size < 1) return set_error(GCCTB_EINVAL, "batch_size must be >= 1");
   if (cfg->rows < 1 || cfg->rows > 0xfffffffeull) return set_error(GCCTB_EINVAL, "rows out of range");
   if (cfg->cols < 1 || cfg->cols > 255) return set_error(GCCTB_EINVAL, "cols out of range");
+  // The adaptive admission window is switched off: with Silo it made a kernel
+  // access memory out of bounds (and once return reads that differ from the
+  // serial replay) in runs whose cause has not been found.
+  if (cfg->persist_adaptive)
+    return set_error(GCCTB_EINVAL, "persist_adaptive is disabled in this build (a memory fault with Silo is not "
+                                   "resolved); use a static persist_window");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_error(GCCTB_EDEVICE, "no HIP device: the transaction kernels need an MI355X (gfx950)");
@@ -664,6 +670,9 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
     if (c->scheme == SCHEME_MVCC) {
       c->pool_cap = cap;
       c->pool.alloc(3 * cap + 3);
+      // nodes are published before they are linked, but a zeroed pool keeps
+      // an unpublished node from ever holding a link that points anywhere
+      HIP_TRY(hipMemsetAsync(c->pool.p, 0, c->pool.n * sizeof(uint64_t), c->stream));
     }
     if (c->scheme == SCHEME_GACCO || c->scheme == SCHEME_GPUTX) {
       // sort buffers for the whole load (the persistent form prepares it at once)

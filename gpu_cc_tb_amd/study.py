@@ -136,8 +136,7 @@ def main(argv=None) -> int:
     b.add_argument("--rows", type=int, default=10485760)
     b.add_argument("--batches", type=int, default=256)
     b.add_argument("--schemes", default=",".join(SCHEMES))
-    b.add_argument("--forms", default="plain,graph,persistent:tuned,persistent:0,persistent:4096,persistent:1024,"
-                                      "persistent:0:a",
+    b.add_argument("--forms", default="plain,graph,persistent:tuned,persistent:0,persistent:4096,persistent:1024",
                    help="form[:window[:a]]; persistent:tuned takes the autotuned stream entry")
     b.set_defaults(fn=stream)
     c = sub.add_parser("serve")

@@ -46,7 +46,8 @@ typedef struct gcctb_config {
   uint32_t persist_window; /* persistent and serving forms: at most this many transactions in
                               flight (0 = as many as the device holds) */
   int32_t persist_adaptive; /* 1: the window adapts to the abort rate while the launch runs,
-                               starting from persist_window */
+                               starting from persist_window.  Disabled in this build: gcctb_create
+                               returns GCCTB_EINVAL for 1 */
 } gcctb_config;
 
 /* Result of one run.  The first block mirrors the reference's Metrics and

@@ -26,7 +26,8 @@ def test_exports_every_declared_symbol():
 
 
 def test_bad_config_is_rejected_before_device_use():
-    for kw in [dict(warp_density=6), dict(block_size=33), dict(block_size=0), dict(batch_size=0)]:
+    for kw in [dict(warp_density=6), dict(block_size=33), dict(block_size=0), dict(batch_size=0),
+               dict(persist_adaptive=True)]:
         with pytest.raises(g.GcctbError) as e:
             g.Context("tpl_nw", rows=16, **kw)
         assert e.value.code == -1

@@ -76,8 +76,14 @@ def test_uneven_batch_stream(scheme):
         assert np.array_equal(results["plain"], results["persistent"])
 
 
+ADAPTIVE_OFF = ("the adaptive window is disabled: with Silo it faulted the GPU (out-of-bounds access) "
+                "in a serving run, cause not found")
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("window,adaptive", [(64, False), (0, True), (256, True)])
+@pytest.mark.parametrize("window,adaptive", [(64, False),
+                                             pytest.param(0, True, marks=pytest.mark.skip(reason=ADAPTIVE_OFF)),
+                                             pytest.param(256, True, marks=pytest.mark.skip(reason=ADAPTIVE_OFF))])
 @pytest.mark.parametrize("case", ["hot", "varlen"])
 @pytest.mark.parametrize("scheme", list(g.SCHEMES))
 def test_persistent_window(scheme, case, window, adaptive):
