@@ -21,18 +21,27 @@
 namespace gcctb {
 namespace dev {
 
-// ---- coherent (agent-scope) accesses --------------------------------------
+// ---- coherent accesses ----------------------------------------------------
+// Loads and stores of shared data (cells, row words, chain heads, version
+// nodes, turn counters) are system-scope relaxed atomics (global_load/store
+// sc0 sc1): a store is acknowledged once performed at memory and a load reads
+// from there, not from a line one XCD's L2 cached earlier.  Read-modify-write
+// atomics are performed at the memory-side coherence point at agent scope.
+// With that, waiting for an access to complete (s_waitcnt) orders it before
+// the next one for every XCD, so publication needs no L2-wide write-back or
+// invalidate (an agent-scope release/acquire fence costs both and ran the
+// low-contention workload 2.6x slower).
 __device__ __forceinline__ uint64_t ld(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ uint32_t ld32(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ void st(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ void st32(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 // Compare-and-swap; on failure `expected` receives the current value.
 __device__ __forceinline__ bool cas(uint64_t* p, uint64_t& expected, uint64_t desired) {
@@ -53,19 +62,21 @@ __device__ __forceinline__ uint64_t fetch_or(uint64_t* p, uint64_t v) {
 }
 
 // ---- ordering between different locations ----------------------------------
-// Agent-scope release / acquire fences.  The coherent accesses above keep each
-// location coherent across the XCDs, but a store acknowledged by this XCD's
-// L2 is not yet performed at memory, and another location may still be served
-// from a line cached earlier: publishing data through a second location (cell
-// then version word, node then chain head, cell then turn counter) needs the
-// L2 write-back of the release fence (buffer_wbl2 sc1 + wait) before the flag,
-// and the reader needs the invalidate of the acquire fence (buffer_inv sc1)
-// between the flag and the data.  A wait for the earlier access alone left a
-// reader able to see a new version word with the old cell behind it.
-// release side: data before the flag that publishes it
-__device__ __forceinline__ void order_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+// s_waitcnt simm16 on gfx9: vmcnt[3:0]+[15:14], expcnt[6:4], lgkmcnt[11:8].
+constexpr int WAIT_VM0 = 0 | (0x7 << 4) | (0xf << 8);
+// release side: data before the flag that publishes it.  Every earlier access
+// of this lane has completed (stores performed at memory, see above) before
+// any later one is issued; the workgroup fence keeps the compiler from
+// moving accesses across it.
+__device__ __forceinline__ void order_release() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+}
 // acquire side: flag before the data it guards
-__device__ __forceinline__ void order_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void order_acquire() {
+  __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
 // ---- wave / group intrinsics ----------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
