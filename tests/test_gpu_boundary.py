@@ -1,0 +1,107 @@
+"""GPU tests of the drop-in boundary: the sorted-array index with arbitrary
+keys against the oracle's Find, transactions resolved through such an index,
+the debug-mode serializability check, and the test_ycsb command line."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gpu_cc_tb_amd as g
+import oracle_lib as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "gpu_cc_tb_amd", "bin", "test_ycsb")
+
+
+def random_index(rows, n, seed):
+    r = np.random.default_rng(seed)
+    keys = r.choice(np.uint64(0xFFFFFFFE), size=n, replace=False).astype(np.uint32)
+    vals = r.permutation(rows)[:n].astype(np.uint32)
+    return keys, vals
+
+
+@pytest.mark.gpu
+def test_probe_custom_index():
+    """Probe of an index over arbitrary keys: hits return the value, misses
+    the index length (Find, include/index.cuh:74-91)."""
+    rows, n = 5000, 3000
+    keys, vals = random_index(rows, n, 11)
+    order = np.argsort(keys, kind="stable")
+    sk, sv = keys[order], vals[order].astype(np.uint64)
+    probes = np.concatenate([keys[:1000], np.random.default_rng(12).integers(0, 2**32 - 1, 1000).astype(np.uint32),
+                             np.array([0, 0xFFFFFFFE], dtype=np.uint32)])
+    print(f"[probe] n={n} probes={probes.size}", flush=True)
+    with g.Context("tpl_nw", rows=rows) as ctx:
+        ctx.seed_table()
+        ctx.build_index(keys, vals)
+        got = ctx.probe(probes)
+    exp = np.array([orc.find(sk, sv, int(k)) for k in probes], dtype=np.uint64)
+    assert np.array_equal(got.astype(np.uint64), exp)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["plain", "persistent"])
+@pytest.mark.parametrize("scheme", ["silo", "mvcc", "tpl_wd", "gacco", "gputx"])
+def test_run_through_custom_index(scheme, form):
+    """Transactions whose keys are arbitrary 32-bit values, resolved to rows
+    through the index: the result equals the run on row keys."""
+    rows, ntx = 2048, 1500
+    w = g.generate(rows, ntx, theta=0.8, read_ratio=0.7, len_min=8, len_max=16)
+    keys = np.random.default_rng(21).choice(np.uint64(0xFFFFFFFE), size=rows, replace=False).astype(np.uint32)
+    row_of_op = g.op_key(w.ops).astype(np.int64)
+    mapped = g.Ops(g.make_ops(keys[row_of_op], g.op_col(w.ops), g.op_is_write(w.ops)), w.tx_off)
+    print(f"[index-run] scheme={scheme} form={form}", flush=True)
+    with g.Context(scheme, rows=rows, warp_density=3, block_size=8, batch_size=500) as ctx:
+        ctx.seed_table()
+        ctx.build_index(keys, np.arange(rows, dtype=np.uint32))
+        ctx.load_ops(mapped)
+        st = ctx.run(form)
+        res = ctx.fetch()
+    assert st["committed"] == ntx and st["index_miss"] == 0
+    init = orc.seed_table(g.SEED, rows, 10)
+    exp_t, exp_r, ties = orc.replay(rows, 10, g.SEED, init, w.ops, w.tx_off, res["order"])
+    assert ties == 0
+    isr = ~g.op_is_write(w.ops)
+    assert np.array_equal(res["reads"][isr], exp_r[isr])
+    assert np.array_equal(res["table"], exp_t)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["tpl_nw", "tictoc", "gputx"])
+def test_check_detects_bad_history(scheme):
+    """gcctb_check (the driver's debug check): clean on the true initial
+    table, and it counts disagreements when given a different one."""
+    rows, ntx = 1024, 2000
+    w = g.generate(rows, ntx, theta=0.9, read_ratio=0.5, len_min=8, len_max=8)
+    print(f"[check] scheme={scheme}", flush=True)
+    with g.Context(scheme, rows=rows, warp_density=3, block_size=8, batch_size=1000) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        ctx.run("plain")
+        init = orc.seed_table(g.SEED, rows, 10)
+        assert ctx.check(init) == (0, 0)
+        wrong = init.copy()
+        wrong[::7] ^= np.uint64(1)
+        bad, unc = ctx.check(wrong)
+    assert bad > 0 and unc == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["silo", "gacco"])
+def test_cli_debug_run(scheme, tmp_path):
+    """The drop-in driver: same argv as the reference's test_ycsb, its TRY
+    line, the debug check's GOOD, and the 15-field CSV line appended to
+    <dataset>.txt."""
+    assert os.path.exists(CLI), "build the tools first (gpu_cc_tb_amd/build.py)"
+    dataset = "gen:theta=0.6,rr=0.5,len=16"
+    args = [CLI, dataset, "4096", "8192", scheme, "3", "8", "2048", "1"]
+    print(f"[cli] {' '.join(args[1:])}", flush=True)
+    p = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert f"TRY {scheme},3,8,2048" in p.stdout
+    assert "SCAN OK" in p.stdout and "GOOD" in p.stdout and "BAD" not in p.stdout
+    line = (tmp_path / (dataset + ".txt")).read_text().strip().splitlines()[-1]
+    fields = line.split(",")
+    assert len(fields) == 15 and fields[0] == scheme
+    assert float(fields[4]) > 0  # throughput
