@@ -176,17 +176,27 @@ def main():
         shape = autotune.lookup(s, args.theta, args.read_ratio, table)
         wd = args.wd if args.wd >= 0 else shape["warp_density"]
         bs = args.bs if args.bs >= 0 else shape["block_size"]
+        # the plain form runs at its own best low-contention shape
+        pwd, pbs = (wd, bs) if args.wd >= 0 or args.bs >= 0 else autotune.DEFAULT_SHAPE[s]
+        plain_here = not args.no_plain and (pwd, pbs) == (wd, bs)
+        sp_ms = 0.0
         with g.Context(s, rows=args.rows, warp_density=wd, block_size=bs, batch_size=args.batch,
                        persist_window=shape["window"], persist_adaptive=shape["adaptive"]) as ctx:
             ctx.seed_table()
             ctx.load_ops(w)
             for _ in range(args.warmup):
                 ctx.run("persistent")
-                if not args.no_plain:
+                if plain_here:
                     ctx.run("plain")
             sb_ms, sb_kern, sb_launch, sb_abort = run_form(ctx, "persistent", args.steps)
-            sp_ms = 0.0
-            if not args.no_plain:
+            if plain_here:
+                sp_ms, _, _, _ = run_form(ctx, "plain", args.steps)
+        if not args.no_plain and not plain_here:
+            with g.Context(s, rows=args.rows, warp_density=pwd, block_size=pbs, batch_size=args.batch) as ctx:
+                ctx.seed_table()
+                ctx.load_ops(w)
+                for _ in range(args.warmup):
+                    ctx.run("plain")
                 sp_ms, _, _, _ = run_form(ctx, "plain", args.steps)
         per[s] = {
             "batched_txps": round(args.txs * args.steps / (sb_ms / 1e3)),
@@ -194,7 +204,7 @@ def main():
             "batched_ms_per_pass": round(sb_ms / args.steps, 3),
             "kernel_ms_per_pass": round(sb_kern / args.steps, 3),
             "aborts_per_pass": sb_abort // args.steps,
-            "wd": wd, "bs": bs, "window": shape["window"],
+            "wd": wd, "bs": bs, "window": shape["window"], "plain_wd": pwd, "plain_bs": pbs,
         }
         b_ms += sb_ms
         b_kern += sb_kern

@@ -26,8 +26,9 @@ hipError_t build_level_from_leaves(uint32_t* dst, const uint32_t* leaves, uint64
                                    uint64_t padded, hipStream_t s);
 hipError_t build_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t nsrc_nodes,
                                   uint64_t padded, hipStream_t s);
+// rows_out[i] = row of ops[i]'s key; also into rows_copy when it is given
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                 hipStream_t s);
+                 hipStream_t s, uint32_t* rows_copy = nullptr);
 
 // --- per-op helpers ---
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s);

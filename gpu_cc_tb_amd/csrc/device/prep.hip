@@ -25,17 +25,30 @@ inline unsigned grid_stride_blocks(uint64_t n) {
   return (unsigned)g;
 }
 
-// Wave reductions (the streaming kernels run full 256-thread blocks).
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) v += dev::shfl_xor64(v, m);
-  return v;
+// Kernels that end in an atomic on one address run at most REDUCE_BLOCKS
+// blocks and combine their waves in LDS first: one address sustains only
+// ~1e8 atomics/s, so one atomic per wave of a 16M-element grid costs ~3 ms.
+constexpr unsigned REDUCE_BLOCKS = 2048;
+inline unsigned reduce_blocks(uint64_t n) {
+  const unsigned g = grid_stride_blocks(n);
+  return g < REDUCE_BLOCKS ? g : REDUCE_BLOCKS;
 }
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+// Block-wide sum (Max = false) or max (Max = true) of one value per thread;
+// the result is valid in thread 0.  Every thread of the block must call it.
+template <bool Max>
+__device__ __forceinline__ uint64_t block_reduce_u64(uint64_t v) {
+  __shared__ uint64_t part[BT / 64];
 #pragma unroll
   for (int m = 1; m < 64; m <<= 1) {
-    const uint32_t o = (uint32_t)__shfl_xor((int)v, m, 64);
-    v = o > v ? o : v;
+    const uint64_t o = dev::shfl_xor64(v, m);
+    v = Max ? (o > v ? o : v) : v + o;
+  }
+  if (dev::lane_id() == 0) part[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = part[0];
+#pragma unroll
+    for (int w = 1; w < BT / 64; ++w) v = Max ? (part[w] > v ? part[w] : v) : v + part[w];
   }
   return v;
 }
@@ -79,10 +92,14 @@ __global__ void k_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t 
     dst[j] = j < nsrc_nodes ? src[j * IDX_FANOUT] : 0xffffffffu;
 }
 
-__global__ void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out) {
+__global__ void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
+                        uint32_t* rows_copy) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    rows_out[i] = dev::index_find(ix, op_key(ops[i]));
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = dev::index_find(ix, op_key(ops[i]));
+    rows_out[i] = r;
+    if (rows_copy) rows_copy[i] = r;
+  }
 }
 
 __global__ void k_op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx) {
@@ -368,10 +385,10 @@ __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const u
         local++;
     }
   }
-  // one atomic per wave: a single counter sustains only ~1e8 atomics/s
+  // one atomic per block (launched with reduce_blocks)
   if (conflicts) {
-    local = wave_sum_u64(local);
-    if (dev::lane_id() == 0 && local) atomicAdd(conflicts, (unsigned long long)local);
+    local = block_reduce_u64<false>(local);
+    if (threadIdx.x == 0 && local) atomicAdd(conflicts, (unsigned long long)local);
   }
 }
 
@@ -395,8 +412,9 @@ __global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_
     if (local) atomicMax(&rank[op_tx[sop[p]]], local);  // ranks start at 0
     mx = local > mx ? local : mx;
   }
-  mx = wave_max_u32(mx);
-  if (dev::lane_id() == 0 && mx) atomicMax(max_rank, mx);
+  // one atomic per block (launched with reduce_blocks)
+  mx = (uint32_t)block_reduce_u64<true>(mx);
+  if (threadIdx.x == 0 && mx) atomicMax(max_rank, mx);
 }
 __global__ void k_gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                                 uint64_t n, int rank_bits, uint64_t* keys) {
@@ -482,6 +500,11 @@ namespace launch {
     hipLaunchKernelGGL(kern, dim3(grid_stride_blocks(n)), dim3(BT), 0, s, __VA_ARGS__); \
     return hipGetLastError();                                                     \
   } while (0)
+#define GC_LAUNCH_REDUCE(kern, n, ...)                                            \
+  do {                                                                            \
+    hipLaunchKernelGGL(kern, dim3(reduce_blocks(n)), dim3(BT), 0, s, __VA_ARGS__); \
+    return hipGetLastError();                                                     \
+  } while (0)
 
 hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s) {
   GC_LAUNCH(k_seed_table, rows * cols, cells, rows, cols, seed);
@@ -499,8 +522,8 @@ hipError_t build_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t n
   GC_LAUNCH(k_level_from_level, padded, dst, src, nsrc_nodes, padded);
 }
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                 hipStream_t s) {
-  GC_LAUNCH(k_probe, n, ix, ops, n, rows_out);
+                 hipStream_t s, uint32_t* rows_copy) {
+  GC_LAUNCH(k_probe, n, ix, ops, n, rows_out, rows_copy);
 }
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s) {
   GC_LAUNCH(k_op_tx, ntx, tx_off, ntx, op_tx);
@@ -543,7 +566,7 @@ hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t
                        uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
                        uint64_t nrows, const uint32_t* op_tx, const uint32_t* tx_rank,
                        const uint64_t* ops, uint64_t* same_order_conflicts, hipStream_t s) {
-  GC_LAUNCH(k_access_need, n, srow, sop, gstart, n, pos_base, need, counters, nrows, op_tx, tx_rank,
+  GC_LAUNCH_REDUCE(k_access_need, n, srow, sop, gstart, n, pos_base, need, counters, nrows, op_tx, tx_rank,
             ops, (unsigned long long*)same_order_conflicts);
 }
 hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
@@ -552,7 +575,7 @@ hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* 
 }
 hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
                     const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s) {
-  GC_LAUNCH(k_gtx_rank, n, sop, S, SS, n, op_tx, rank, max_rank);
+  GC_LAUNCH_REDUCE(k_gtx_rank, n, sop, S, SS, n, op_tx, rank, max_rank);
 }
 hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                          uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s) {

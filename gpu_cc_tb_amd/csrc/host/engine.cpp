@@ -166,6 +166,9 @@ class SchemeHost {
     (void)o1;
     HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream));
   }
+  // True when PrepStream / PrepLaunch leave the row of every op they cover in
+  // rows_out, so the persistent and serving forms need no probe pass of their own.
+  virtual bool ProbesRows() const { return false; }
   virtual void Init(int batch_id, uint64_t batch_st) {
     (void)batch_id;
     (void)batch_st;
@@ -281,12 +284,14 @@ class GaccoHost : public SchemeHost {
   void PrepStream() override { prep_range(0, ctx.nops); }
   void PostStream() override {}
   void PrepLaunch(uint64_t o0, uint64_t o1) override { prep_range(o0, o1); }
-  // per-row access queues for ops [o0, o1) in (row, transaction id) order
+  bool ProbesRows() const override { return true; }
+  // per-row access queues for ops [o0, o1) in (row, transaction id) order;
+  // the rows also land in rows_out[o0, o1)
   void prep_range(uint64_t o0, uint64_t o1) {
     const uint64_t n = o1 - o0;
     if (n == 0) return;
     hipStream_t s = ctx.stream;
-    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.sk0.p, s));
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.rows_out.p + o0, s, ctx.sk0.p));
     HIP_TRY(launch::iota_u32(ctx.sv0.p, n, (uint32_t)o0, s));
     uint32_t *srow, *sop;
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.ix.length),
@@ -314,8 +319,7 @@ class GPUTxHost : public SchemeHost {
     const uint64_t n = ctx.nops, ntx = ctx.ntx;
     hipStream_t s = ctx.stream;
     // per-row access lists in (row, tx) order
-    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s));
-    HIP_TRY(hipMemcpyAsync(ctx.sk0.p, ctx.rows_out.p, n * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p));
     HIP_TRY(launch::iota_u32(ctx.sv0.p, n, 0, s));
     uint32_t *srow, *sop;
     const int row_bits = bits_for(ctx.ix.length);
@@ -374,6 +378,7 @@ class GPUTxHost : public SchemeHost {
   void Init(int, uint64_t) override {}
   // the persistent form runs every rank level in one launch
   void PrepStream() override { Init0(); }
+  bool ProbesRows() const override { return true; }
   void PostStream() override {}
   void Launch(int, uint64_t lo, uint64_t hi, bool persist) override {
     ExecArgs a = args(0, lo, hi);
@@ -765,10 +770,10 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (form == FORM_PERSIST) {
       c->host->PrepStream();
-      // Every op's row in one probe pass (GPUTx's ranking already made it):
-      // a group loading its next transaction then reads rows instead of
-      // walking the index while the other groups of its wave wait.
-      if (c->scheme != SCHEME_GPUTX && c->nops)
+      // Every op's row in one probe pass (GaccO's and GPUTx's preparation
+      // already made it): a group loading its next transaction then reads
+      // rows instead of walking the index while the other groups of its wave wait.
+      if (!c->host->ProbesRows() && c->nops)
         HIP_TRY(launch::probe(c->ix, c->ops.p, c->nops, c->rows_out.p, s));
       HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
@@ -1028,7 +1033,8 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
       const auto tl = clk::now();
       if (o1 > o0) HIP_TRY(hipMemcpyAsync(c->ops.p + o0, host_ops.p + o0, (o1 - o0) * 8, hipMemcpyHostToDevice, s));
       c->host->PrepLaunch(o0, o1);
-      if (o1 > o0) HIP_TRY(launch::probe(c->ix, c->ops.p + o0, o1 - o0, c->rows_out.p + o0, s));
+      if (o1 > o0 && !c->host->ProbesRows())
+        HIP_TRY(launch::probe(c->ix, c->ops.p + o0, o1 - o0, c->rows_out.p + o0, s));
       HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
       c->rows_probed = true;
       c->host->Launch((int)launches, lo, hi, true);
