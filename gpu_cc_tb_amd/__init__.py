@@ -124,6 +124,7 @@ def load_library() -> C.CDLL:
         "gcctb_check": (C.c_int, [P, _u64p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "gcctb_microbench": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]),
+        "gcctb_sort_pairs": (C.c_int, [P, P, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -230,6 +231,25 @@ def microbench(kind: str, n_words: int, threads: int = 1 << 20, iters: int = 16)
     _check(lib.gcctb_microbench(MICROBENCH[kind], n_words, threads, iters, C.byref(rate), C.byref(ms)))
     return {"kind": kind, "n_words": n_words, "threads": threads, "iters": iters, "rate": rate.value,
             "ms": ms.value}
+
+
+def sort_pairs(keys: np.ndarray, vals: np.ndarray, key_bits: int | None = None) -> tuple[np.ndarray, np.ndarray, float]:
+    """Stable device sort of (key, value) pairs by the low `key_bits` bits of
+    the key (all of them by default): the sort behind the GaccO and GPUTx
+    preparation.  Returns sorted copies and the device time in ms."""
+    keys = np.ascontiguousarray(keys)
+    if keys.dtype not in (np.uint32, np.uint64):
+        raise TypeError("keys must be uint32 or uint64")
+    k = keys.copy()
+    v = np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    if v.shape != k.shape or k.ndim != 1:
+        raise ValueError("keys and vals must be 1-D and of one length")
+    nbits = k.itemsize * 8 if key_bits is None else key_bits
+    lib = load_library()
+    ms = C.c_double(0)
+    _check(lib.gcctb_sort_pairs(k.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), k.size, k.itemsize,
+                                nbits, C.byref(ms)))
+    return k, v, ms.value
 
 
 # ---------------------------------------------------------------------------

@@ -33,6 +33,9 @@ hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t*
 // --- per-op helpers ---
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s);
 hipError_t iota_u32(uint32_t* v, uint64_t n, uint32_t base, hipStream_t s);
+// out[i] = (base + i) | OPID_WRITE if ops[base + i] writes: the values the
+// GaccO / GPUTx sorts carry, so later passes read the flag in order.
+hipError_t op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out, hipStream_t s);
 hipError_t fill_u32(uint32_t* v, uint64_t n, uint32_t x, hipStream_t s);
 hipError_t fill_u64(uint64_t* v, uint64_t n, uint64_t x, hipStream_t s);
 
@@ -55,20 +58,24 @@ hipError_t inclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint
 hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
 
 // --- ordered-execution preprocessing ---
-// From accesses sorted by (row, order): group-start flags (p or 0).
-hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
-                              uint64_t n, uint32_t* flags, hipStream_t s);
-// need[sop[p]] = gstart[p] + pos_base; counters[srow[p]] = p + pos_base at row starts.
-// same_order_conflicts counts adjacent same-key conflicting pairs (okey may be null).
+// From accesses sorted by (row, order), sop holding op_ids values: group-start
+// flags (p or 0).
+hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags,
+                              hipStream_t s);
+// need[op] = gstart[p] + pos_base for op = sop[p] & OPID_MASK;
+// counters[srow[p]] = p + pos_base at row starts.
 hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
                        uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
-                       uint64_t nrows, const uint32_t* op_tx, const uint32_t* tx_rank,
-                       const uint64_t* ops, uint64_t* same_order_conflicts, hipStream_t s);
+                       uint64_t nrows, hipStream_t s);
+// GPUTx: *conflicts += adjacent same-row, same-rank accesses where either
+// writes, from the (row << rank_bits | rank) sorted keys and the group flags.
+hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
+                         hipStream_t s);
 // GPUTx one-pass rank (src/gputx.cu:5-27).  boundary[p] = 1 if p is not the
 // first access of its row and (p-1 or p) is a write; rowstart[p] = p at row
 // starts else 0.
-hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
-                     uint64_t nrows, uint32_t* boundary, uint32_t* rowstart, hipStream_t s);
+hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint64_t nrows, uint32_t* boundary,
+                     uint32_t* rowstart, hipStream_t s);
 // local rank = S[p] - S[SS[p]]; rank[tx] = max; max_rank = max.
 hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
                     const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s);

@@ -92,13 +92,82 @@ __global__ void k_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t 
     dst[j] = j < nsrc_nodes ? src[j * IDX_FANOUT] : 0xffffffffu;
 }
 
-__global__ void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                        uint32_t* rows_copy) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = dev::index_find(ix, op_key(ops[i]));
-    rows_out[i] = r;
-    if (rows_copy) rows_copy[i] = r;
+// Quad (4 consecutive lanes) primitives over DPP quad permutes; every lane
+// of the quad must be active.
+template <int J>
+__device__ __forceinline__ uint32_t quad_get(uint32_t v) {  // lane J's value
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, J * 0x55, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t quad_sum(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);  // lanes 1,0,3,2
+  v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);  // lanes 2,3,0,1
+  return v;
+}
+__device__ __forceinline__ int quad_max(int v) {
+  int o = __builtin_amdgcn_mov_dpp(v, 0xb1, 0xf, 0xf, false);
+  v = o > v ? o : v;
+  o = __builtin_amdgcn_mov_dpp(v, 0x4e, 0xf, 0xf, false);
+  return o > v ? o : v;
+}
+
+// Index probe (Find, include/index.cuh:74-91, over the directory layout of
+// dev::index_find).  A quad walks four probes at once: lane q loads op p0+q
+// (a wave loads 64 consecutive ops), the quad shares the four keys, and at
+// every level lane q reads the q-th 16 bytes of each probe's 64-byte node, so
+// a load instruction touches one line per probe instead of four and four
+// independent walks are in flight per quad.  Loads are unconditional (a
+// finished or missing probe reads node 0), so the four issue back to back.
+__global__ __launch_bounds__(256) void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
+                                               uint32_t* rows_copy) {
+  const uint32_t q = threadIdx.x & 3u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of 4
+  for (uint64_t p0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~3ull; p0 < n; p0 += stride) {
+    const uint64_t mine = p0 + q;
+    const uint32_t mykey = mine < n ? op_key(ops[mine]) : 0xffffffffu;
+    const uint32_t key[4] = {quad_get<0>(mykey), quad_get<1>(mykey), quad_get<2>(mykey), quad_get<3>(mykey)};
+    uint32_t node[4] = {0u, 0u, 0u, 0u};
+    bool live[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) live[j] = key[j] != 0xffffffffu;  // padding value: never a key
+    for (int l = (int)ix.nlevels - 1; l >= 0; --l) {
+      const uint4* lv = reinterpret_cast<const uint4*>(ix.level[l]);
+      uint4 v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = lv[(size_t)node[j] * 4 + q];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t c = quad_sum((v[j].x <= key[j]) + (v[j].y <= key[j]) + (v[j].z <= key[j]) +
+                                    (v[j].w <= key[j]));
+        if (live[j] && c == 0) live[j] = false;
+        node[j] = live[j] ? node[j] * IDX_FANOUT + (c - 1) : 0u;
+      }
+    }
+    // leaf: lanes 0,1 hold keys 0-3, 4-7; lanes 2,3 the values 0-3, 4-7
+    const uint4* lf = reinterpret_cast<const uint4*>(ix.leaves);
+    uint4 lv4[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) lv4[j] = lf[(size_t)node[j] * 4 + q];
+    uint32_t res[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint4 v = lv4[j];
+      // the last matching slot, as a scan of the leaf would find
+      const bool look = live[j] && q < 2;
+      int h = -1;
+      h = look && v.x == key[j] ? (int)q * 4 : h;
+      h = look && v.y == key[j] ? (int)q * 4 + 1 : h;
+      h = look && v.z == key[j] ? (int)q * 4 + 2 : h;
+      h = look && v.w == key[j] ? (int)q * 4 + 3 : h;
+      h = quad_max(h);
+      const uint32_t comp = (h & 3) == 0 ? v.x : (h & 3) == 1 ? v.y : (h & 3) == 2 ? v.z : v.w;
+      const uint32_t from2 = quad_get<2>(comp), from3 = quad_get<3>(comp);
+      res[j] = h < 0 ? IDX_MISS : (h >> 2) ? from3 : from2;
+    }
+    const uint32_t r = q == 0 ? res[0] : q == 1 ? res[1] : q == 2 ? res[2] : res[3];
+    if (mine < n) {
+      rows_out[mine] = r;
+      if (rows_copy) rows_copy[mine] = r;
+    }
   }
 }
 
@@ -108,6 +177,13 @@ __global__ void k_op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx) {
     const uint64_t b = tx_off[t], e = tx_off[t + 1];
     for (uint64_t i = b; i < e; ++i) op_tx[i] = (uint32_t)t;
   }
+}
+// Op ids for the GaccO / GPUTx sorts: base + i with the op's write flag in
+// bit 31, so the passes over the sorted order read it without a random load.
+__global__ void k_op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (base + (uint32_t)i) | (op_is_write(ops[base + i]) ? OPID_WRITE : 0u);
 }
 __global__ void k_iota(uint32_t* v, uint64_t n, uint32_t base) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -126,55 +202,73 @@ __global__ void k_fill64(uint64_t* v, uint64_t n, uint64_t x) {
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort.  A tile is RS_ITEMS x RS_BT elements; element e of
-// the tile is handled in iteration e / RS_BT by thread e % RS_BT, so
-// (iteration, thread) order is input order and the in-tile rank is stable.
-// Within a wave, lanes with equal digits are found with 8 ballots; waves are
-// combined through per-digit LDS counters.
+// Stable LSD radix sort, 8 bits per pass: per-tile digit histograms, one scan
+// of the [digit][tile] matrix, then the scatter.  A tile is RS_TILE elements;
+// wave w of the block holds tile elements [w*RS_WSPAN, (w+1)*RS_WSPAN),
+// lane-minor, so (wave, item, lane) order is input order.  The scatter ranks
+// every element within its digit (ballots inside a wave, a per-wave LDS
+// counter across its items, a prefix over waves), sorts the tile in LDS, and
+// writes it out digit run by digit run, so consecutive lanes store to
+// consecutive addresses.
 // ---------------------------------------------------------------------------
 constexpr int RS_BT = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_BT * RS_ITEMS;
 constexpr int RS_WAVES = RS_BT / 64;
+constexpr int RS_WSPAN = RS_ITEMS * 64;
 
 template <class KT>
-__global__ __launch_bounds__(RS_BT) void k_rs_hist(const KT* keys, uint64_t n, int shift,
+__global__ __launch_bounds__(RS_BT) void k_rs_hist(const KT* keys, uint64_t n, int shift, uint32_t dmask,
                                                    uint32_t* hist, uint32_t nb) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
-  __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  uint32_t d[RS_ITEMS];
+#pragma unroll
   for (int i = 0; i < RS_ITEMS; ++i) {
     const uint64_t e = base + (uint64_t)i * RS_BT + threadIdx.x;
-    if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xffu], 1u);
+    d[i] = e < n ? (uint32_t)(keys[e] >> shift) & dmask : 256u;
   }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; ++i)
+    if (d[i] < 256u) atomicAdd(&h[d[i]], 1u);
   __syncthreads();
   hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
+// offs[d * nb + b]: where tile b's first element of digit d goes; the digit
+// is (key >> shift) & dmask (dmask < 0xff only on a last, narrower pass)
 template <class KT>
 __global__ __launch_bounds__(RS_BT) void k_rs_scatter(const KT* kin, const uint32_t* vin, KT* kout,
-                                                      uint32_t* vout, uint64_t n, int shift,
+                                                      uint32_t* vout, uint64_t n, int shift, uint32_t dmask,
                                                       const uint32_t* offs, uint32_t nb) {
-  __shared__ uint32_t gofs[256];
-  __shared__ uint32_t wcnt[RS_WAVES][256];
-  gofs[threadIdx.x] = offs[(uint64_t)threadIdx.x * nb + blockIdx.x];
-  const uint32_t lane = dev::lane_id();
-  const uint32_t wave = threadIdx.x / 64;
+  __shared__ KT skey[RS_TILE];
+  __shared__ uint32_t sval[RS_TILE];
+  __shared__ uint32_t wcnt[RS_WAVES][256];  // digit counts per wave, then each wave's start in the digit
+  __shared__ uint32_t dstart[256];          // tile-local start of each digit
+  __shared__ uint32_t gbase[256];           // global start of each digit's run from this tile
+  __shared__ uint32_t wsum[RS_WAVES];
+  const uint32_t t = threadIdx.x, lane = dev::lane_id(), wave = t / 64;
   const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
-  for (int i = 0; i < RS_ITEMS; ++i) {
+  const uint32_t cnt = n - base < (uint64_t)RS_TILE ? (uint32_t)(n - base) : (uint32_t)RS_TILE;
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) wcnt[w][threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t e = base + (uint64_t)i * RS_BT + threadIdx.x;
-    const bool valid = e < n;
-    KT key = 0;
-    uint32_t val = 0, d = 0;
-    if (valid) {
-      key = kin[e];
-      val = vin[e];
-      d = (uint32_t)(key >> shift) & 0xffu;
-    }
+  for (int w = 0; w < RS_WAVES; ++w) wcnt[w][t] = 0;
+  KT k[RS_ITEMS];
+  uint32_t v[RS_ITEMS];
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; ++i) {
+    const uint32_t e = wave * RS_WSPAN + i * 64 + lane;
+    k[i] = e < cnt ? kin[base + e] : (KT)0;
+    v[i] = e < cnt ? vin[base + e] : 0u;
+  }
+  __syncthreads();
+  // rank within (wave, digit): earlier items of the wave, then lower lanes
+  uint32_t r[RS_ITEMS];
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; ++i) {
+    const bool valid = wave * RS_WSPAN + i * 64 + lane < cnt;
+    const uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
     uint64_t peers = dev::ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -183,27 +277,52 @@ __global__ __launch_bounds__(RS_BT) void k_rs_scatter(const KT* kin, const uint3
       peers &= bit ? bb : ~bb;
     }
     const uint32_t below = (uint32_t)__popcll(peers & dev::lanemask_lt());
-    if (valid && below == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      // exclusive prefix over waves for digit t, then advance the tile base
-      const uint32_t t = threadIdx.x;
-      uint32_t run = gofs[t];
+    uint32_t old = 0;
+    if (valid && below == 0) old = atomicAdd(&wcnt[wave][d], (uint32_t)__popcll(peers));
+    const int leader = peers ? __ffsll((unsigned long long)peers) - 1 : (int)lane;
+    r[i] = (uint32_t)__shfl((int)old, leader, 64) + below;
+  }
+  __syncthreads();
+  // digit t: each wave's start within the digit, the digit's tile total, and
+  // the tile-local start of the digit (exclusive scan over digits)
+  uint32_t tot = 0;
 #pragma unroll
-      for (int w = 0; w < RS_WAVES; ++w) {
-        const uint32_t c = wcnt[w][t];
-        wcnt[w][t] = run;
-        run += c;
-      }
-      gofs[t] = run;
+  for (int w = 0; w < RS_WAVES; ++w) {
+    const uint32_t c = wcnt[w][t];
+    wcnt[w][t] = tot;
+    tot += c;
+  }
+  uint32_t x = tot;
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)x, dd, 64);
+    if (lane >= (uint32_t)dd) x += o;
+  }
+  if (lane == 63) wsum[wave] = x;
+  gbase[t] = offs[(uint64_t)t * nb + blockIdx.x];
+  __syncthreads();
+  uint32_t carry = 0;
+  for (uint32_t w = 0; w < wave; ++w) carry += wsum[w];
+  dstart[t] = carry + x - tot;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; ++i) {
+    if (wave * RS_WSPAN + i * 64 + lane < cnt) {
+      const uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
+      const uint32_t lp = dstart[d] + wcnt[wave][d] + r[i];
+      skey[lp] = k[i];
+      sval[lp] = v[i];
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t pos = wcnt[wave][d] + below;
+  }
+  __syncthreads();
+  for (uint32_t j = t; j < cnt; j += RS_BT) {
+    const KT key = skey[j];
+    const uint32_t d = (uint32_t)(key >> shift) & dmask;
+    const uint32_t pos = gbase[d] + (j - dstart[d]);
+    if (pos < n) {  // always true when the counts agree; keeps a logic error off foreign memory
       kout[pos] = key;
-      vout[pos] = val;
+      vout[pos] = sval[j];
     }
-    __syncthreads();
   }
 }
 
@@ -334,12 +453,13 @@ hipError_t radix_impl(KT* k0, uint32_t* v0, KT* k1, uint32_t* v1, uint64_t n, in
     uint32_t* hist = tmp;
     uint32_t* stmp = tmp + 256 * nb;
     for (int shift = 0; shift < key_bits; shift += 8) {
-      hipLaunchKernelGGL((k_rs_hist<KT>), dim3((unsigned)nb), dim3(RS_BT), 0, s, ka, n, shift, hist,
+      const uint32_t dmask = key_bits - shift >= 8 ? 0xffu : (1u << (key_bits - shift)) - 1u;
+      hipLaunchKernelGGL((k_rs_hist<KT>), dim3((unsigned)nb), dim3(RS_BT), 0, s, ka, n, shift, dmask, hist,
                          (uint32_t)nb);
       hipError_t e = scan_impl<OpAdd, false>(hist, hist, 256 * nb, stmp, s);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL((k_rs_scatter<KT>), dim3((unsigned)nb), dim3(RS_BT), 0, s, ka, va, kb, vb, n,
-                         shift, hist, (uint32_t)nb);
+                         shift, dmask, hist, (uint32_t)nb);
       KT* tk = ka;
       ka = kb;
       kb = tk;
@@ -356,51 +476,49 @@ hipError_t radix_impl(KT* k0, uint32_t* v0, KT* k1, uint32_t* v1, uint64_t n, in
 // ---------------------------------------------------------------------------
 // Ordered-execution passes.
 // ---------------------------------------------------------------------------
-__global__ void k_access_group_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
-                                     uint64_t n, uint32_t* flags) {
+__global__ void k_access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
     bool start = (p == 0) || srow[p] != srow[p - 1];
-    if (!start) start = op_is_write(ops[sop[p]]) || op_is_write(ops[sop[p - 1]]);
+    if (!start) start = ((sop[p] | sop[p - 1]) & OPID_WRITE) != 0u;
     flags[p] = start ? (uint32_t)p : 0u;
   }
 }
 __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
                               uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
-                              uint64_t nrows, const uint32_t* op_tx, const uint32_t* tx_rank,
-                              const uint64_t* ops, unsigned long long* conflicts) {
-  uint64_t local = 0;
+                              uint64_t nrows) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t op = sop[p];
-    need[op] = gstart[p] + pos_base;
-    const bool row_start = (p == 0) || srow[p] != srow[p - 1];
-    if (row_start) {
-      // an index miss probes to the index length; such ops never execute
-      if (srow[p] < nrows) counters[srow[p]] = (uint32_t)p + pos_base;
-    } else if (tx_rank && conflicts) {
-      const uint32_t pop = sop[p - 1];
-      if ((op_is_write(ops[op]) || op_is_write(ops[pop])) &&
-          tx_rank[op_tx[op]] == tx_rank[op_tx[pop]])
-        local++;
-    }
-  }
-  // one atomic per block (launched with reduce_blocks)
-  if (conflicts) {
-    local = block_reduce_u64<false>(local);
-    if (threadIdx.x == 0 && local) atomicAdd(conflicts, (unsigned long long)local);
+    need[sop[p] & OPID_MASK] = gstart[p] + pos_base;
+    // an index miss probes to the index length; such ops never execute
+    if (((p == 0) || srow[p] != srow[p - 1]) && srow[p] < nrows) counters[srow[p]] = (uint32_t)p + pos_base;
   }
 }
 
-__global__ void k_gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
-                            uint64_t nrows, uint32_t* boundary, uint32_t* rowstart) {
+// GPUTx same-rank conflicts: adjacent accesses of one row with equal rank
+// where either writes.  In the (row << rank_bits | rank) sorted keys, same
+// row and rank means equal keys; flags[p] (access_group_flags) is non-zero
+// at a non-row-start position exactly when either access writes.
+__global__ void k_gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n,
+                                unsigned long long* conflicts) {
+  uint64_t local = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x)
+    local += (skey[p] == skey[p - 1] && flags[p] != 0u) ? 1u : 0u;
+  // one atomic per block (launched with reduce_blocks)
+  local = block_reduce_u64<false>(local);
+  if (threadIdx.x == 0 && local) atomicAdd(conflicts, (unsigned long long)local);
+}
+
+__global__ void k_gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint64_t nrows,
+                            uint32_t* boundary, uint32_t* rowstart) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const bool start = (p == 0) || srow[p] != srow[p - 1];
     rowstart[p] = start ? (uint32_t)p : 0u;
     // index misses (IDX_MISS sorts last) never execute and raise no rank
     boundary[p] = (!start && srow[p] < nrows &&
-                   (op_is_write(ops[sop[p]]) || op_is_write(ops[sop[p - 1]]))) ? 1u : 0u;
+                   ((sop[p] | sop[p - 1]) & OPID_WRITE) != 0u) ? 1u : 0u;
   }
 }
 __global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
@@ -409,7 +527,7 @@ __global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t local = S[p] - S[SS[p]];
-    if (local) atomicMax(&rank[op_tx[sop[p]]], local);  // ranks start at 0
+    if (local) atomicMax(&rank[op_tx[sop[p] & OPID_MASK]], local);  // ranks start at 0
     mx = local > mx ? local : mx;
   }
   // one atomic per block (launched with reduce_blocks)
@@ -528,6 +646,9 @@ hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t*
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s) {
   GC_LAUNCH(k_op_tx, ntx, tx_off, ntx, op_tx);
 }
+hipError_t op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out, hipStream_t s) {
+  GC_LAUNCH(k_op_ids, n, ops, n, base, out);
+}
 hipError_t iota_u32(uint32_t* v, uint64_t n, uint32_t base, hipStream_t s) {
   GC_LAUNCH(k_iota, n, v, n, base);
 }
@@ -558,20 +679,22 @@ hipError_t inclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint
 hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
   return scan_impl<OpMax, true>(in, out, n, tmp, s);
 }
-hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
-                              uint64_t n, uint32_t* flags, hipStream_t s) {
-  GC_LAUNCH(k_access_group_flags, n, srow, sop, ops, n, flags);
+hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags,
+                              hipStream_t s) {
+  GC_LAUNCH(k_access_group_flags, n, srow, sop, n, flags);
 }
 hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
                        uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
-                       uint64_t nrows, const uint32_t* op_tx, const uint32_t* tx_rank,
-                       const uint64_t* ops, uint64_t* same_order_conflicts, hipStream_t s) {
-  GC_LAUNCH_REDUCE(k_access_need, n, srow, sop, gstart, n, pos_base, need, counters, nrows, op_tx, tx_rank,
-            ops, (unsigned long long*)same_order_conflicts);
+                       uint64_t nrows, hipStream_t s) {
+  GC_LAUNCH(k_access_need, n, srow, sop, gstart, n, pos_base, need, counters, nrows);
 }
-hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, const uint64_t* ops, uint64_t n,
-                     uint64_t nrows, uint32_t* boundary, uint32_t* rowstart, hipStream_t s) {
-  GC_LAUNCH(k_gtx_flags, n, srow, sop, ops, n, nrows, boundary, rowstart);
+hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
+                         hipStream_t s) {
+  GC_LAUNCH_REDUCE(k_gtx_conflicts, n, skey, flags, n, (unsigned long long*)conflicts);
+}
+hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint64_t nrows, uint32_t* boundary,
+                     uint32_t* rowstart, hipStream_t s) {
+  GC_LAUNCH(k_gtx_flags, n, srow, sop, n, nrows, boundary, rowstart);
 }
 hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
                     const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s) {
@@ -593,6 +716,17 @@ hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks, uint32
 }
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
                       uint32_t* vprev, hipStream_t s) {
+  // Outside a run every CC word and chain head is zero, so clearing the whole
+  // arrays leaves the same state.  That streams rows x 12 bytes instead of one
+  // scattered line per op: the cheaper form once n >= rows / 8.
+  if (n * 8 >= nrows_table) {
+    if (cc) {
+      const hipError_t e = fill_u64(cc, nrows_table, 0, s);
+      if (e != hipSuccess) return e;
+    }
+    if (vprev) return fill_u32(vprev, nrows_table, 0, s);
+    return hipSuccess;
+  }
   GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, cc, vprev);
 }
 hipError_t adapt_begin(uint64_t* ctrs, uint64_t window, hipStream_t s) {

@@ -49,6 +49,10 @@ GC_HD uint64_t make_op(uint32_t key, uint32_t col, bool is_write) {
 GC_HD uint32_t op_key(uint64_t op) { return (uint32_t)op; }
 GC_HD uint32_t op_col(uint64_t op) { return (uint32_t)(op >> 32) & 0xffu; }
 GC_HD bool op_is_write(uint64_t op) { return (op >> 40) & 1u; }
+// An op id (index into the op list, below 2^31) with the op's write flag in
+// bit 31: the value the GaccO / GPUTx preparation sorts carry.
+constexpr uint32_t OPID_WRITE = 0x80000000u;
+constexpr uint32_t OPID_MASK = 0x7fffffffu;
 
 // ---------------------------------------------------------------------------
 // Deterministic hashing.  splitmix64 (Steele, Lea, Flood: "Fast splittable

@@ -292,14 +292,13 @@ class GaccoHost : public SchemeHost {
     if (n == 0) return;
     hipStream_t s = ctx.stream;
     HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.rows_out.p + o0, s, ctx.sk0.p));
-    HIP_TRY(launch::iota_u32(ctx.sv0.p, n, (uint32_t)o0, s));
+    HIP_TRY(launch::op_ids(ctx.ops.p, n, (uint32_t)o0, ctx.sv0.p, s));
     uint32_t *srow, *sop;
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.ix.length),
                                    ctx.sort_tmp.p, &srow, &sop, s));
-    HIP_TRY(launch::access_group_flags(srow, sop, ctx.ops.p, n, ctx.flags.p, s));
+    HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s));
     HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
-    HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows,
-                                nullptr, nullptr, ctx.ops.p, nullptr, s));
+    HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows, s));
   }
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
     HIP_TRY(launch::exec_ordered(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
@@ -320,13 +319,13 @@ class GPUTxHost : public SchemeHost {
     hipStream_t s = ctx.stream;
     // per-row access lists in (row, tx) order
     HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p));
-    HIP_TRY(launch::iota_u32(ctx.sv0.p, n, 0, s));
+    HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
     uint32_t *srow, *sop;
     const int row_bits = bits_for(ctx.ix.length);
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, row_bits, ctx.sort_tmp.p,
                                    &srow, &sop, s));
     // one-pass local ranks and rank[tx] = max (src/gputx.cu:5-27)
-    HIP_TRY(launch::gtx_flags(srow, sop, ctx.ops.p, n, ctx.cfg.rows, ctx.flags.p, ctx.gstart.p, s));
+    HIP_TRY(launch::gtx_flags(srow, sop, n, ctx.cfg.rows, ctx.flags.p, ctx.gstart.p, s));
     HIP_TRY(launch::inclusive_sum_u32(ctx.flags.p, ctx.S.p, n, ctx.sort_tmp.p, s));
     HIP_TRY(launch::inclusive_max_u32(ctx.gstart.p, ctx.SS.p, n, ctx.sort_tmp.p, s));
     HIP_TRY(launch::fill_u32(ctx.rank.p, ntx, 0, s));
@@ -362,17 +361,18 @@ class GPUTxHost : public SchemeHost {
     // execution order of every row: (row, rank, tx)
     if (n) {
       HIP_TRY(launch::gtx_exec_keys(ctx.rows_out.p, ctx.op_tx.p, ctx.rank.p, n, rank_bits, ctx.sk64a.p, s));
-      HIP_TRY(launch::iota_u32(ctx.sv0.p, n, 0, s));
+      HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
       uint64_t* skey;
       uint32_t* sop2;
       HIP_TRY(launch::radix_sort_u64(ctx.sk64a.p, ctx.sv0.p, ctx.sk64b.p, ctx.sv1.p, n, row_bits + rank_bits,
                                      ctx.sort_tmp.p, &skey, &sop2, s));
       HIP_TRY(launch::u64_key_high(skey, n, rank_bits, ctx.sk0.p, s));
-      HIP_TRY(launch::access_group_flags(ctx.sk0.p, sop2, ctx.ops.p, n, ctx.flags.p, s));
+      HIP_TRY(launch::access_group_flags(ctx.sk0.p, sop2, n, ctx.flags.p, s));
       HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
-      HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
       HIP_TRY(launch::access_need(ctx.sk0.p, sop2, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p,
-                                  ctx.cfg.rows, ctx.op_tx.p, ctx.rank.p, ctx.ops.p, ctx.conflicts.p, s));
+                                  ctx.cfg.rows, s));
+      HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
+      HIP_TRY(launch::gtx_conflicts(skey, ctx.flags.p, n, ctx.conflicts.p, s));
     }
   }
   void Init(int, uint64_t) override {}
@@ -641,7 +641,8 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
     }
   }
   const uint64_t nops = tx_off[ntx];
-  if (nops >= (1ull << 32)) return set_error(GCCTB_EINVAL, "at most 2^32-1 ops");
+  // op ids carry a write flag in bit 31 through the GaccO / GPUTx sorts
+  if (nops >= (1ull << 31)) return set_error(GCCTB_EINVAL, "at most 2^31-1 ops");
   // MVCC version pool: the largest number of writes in one batch
   uint64_t cap = 0;
   // MVCC version pool: one node per write of the whole load (the persistent
@@ -1220,6 +1221,63 @@ int gcctb_check(gcctb_ctx* c, const uint64_t* init_table, uint64_t* bad, uint64_
                 c->h_tx_off.data(), c->ntx, order.data(), reads.data(), bad);
   if (uncommitted) *uncommitted = c->ntx - std::min(c->ntx, c->last_committed);
   return GCCTB_OK;
+}
+
+int gcctb_sort_pairs(void* keys, uint32_t* vals, uint64_t n, int32_t key_bytes, int32_t key_bits, double* ms) {
+  if ((key_bytes != 4 && key_bytes != 8) || key_bits < 0 || key_bits > key_bytes * 8 || (n && (!keys || !vals)))
+    return set_error(GCCTB_EINVAL, "bad sort arguments");
+  if (n > 0xffffffffull) return set_error(GCCTB_EINVAL, "the sort takes at most 2^32 - 1 pairs");
+  if (ms) *ms = 0;
+  if (n == 0) return GCCTB_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(GCCTB_EDEVICE, "no HIP device");
+  hipStream_t s = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = GCCTB_OK;
+  try {
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    DevBuf<uint32_t> v0, v1, tmp;
+    v0.alloc(n);
+    v1.alloc(n);
+    tmp.alloc(launch::radix_sort_tmp_words(n));
+    HIP_TRY(hipMemcpyAsync(v0.p, vals, n * 4, hipMemcpyHostToDevice, s));
+    uint32_t* vout = nullptr;
+    if (key_bytes == 4) {
+      DevBuf<uint32_t> k0, k1;
+      k0.alloc(n);
+      k1.alloc(n);
+      uint32_t* kout = nullptr;
+      HIP_TRY(hipMemcpyAsync(k0.p, keys, n * 4, hipMemcpyHostToDevice, s));
+      HIP_TRY(hipEventRecord(ev[0], s));
+      HIP_TRY(launch::radix_sort_u32(k0.p, v0.p, k1.p, v1.p, n, key_bits, tmp.p, &kout, &vout, s));
+      HIP_TRY(hipEventRecord(ev[1], s));
+      HIP_TRY(hipMemcpyAsync(keys, kout, n * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(vals, vout, n * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+    } else {
+      DevBuf<uint64_t> k0, k1;
+      k0.alloc(n);
+      k1.alloc(n);
+      uint64_t* kout = nullptr;
+      HIP_TRY(hipMemcpyAsync(k0.p, keys, n * 8, hipMemcpyHostToDevice, s));
+      HIP_TRY(hipEventRecord(ev[0], s));
+      HIP_TRY(launch::radix_sort_u64(k0.p, v0.p, k1.p, v1.p, n, key_bits, tmp.p, &kout, &vout, s));
+      HIP_TRY(hipEventRecord(ev[1], s));
+      HIP_TRY(hipMemcpyAsync(keys, kout, n * 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(vals, vout, n * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+    }
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+    if (ms) *ms = t;
+  } catch (const HipFailure& f) {
+    rc = hip_fail(f);
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s) (void)hipStreamDestroy(s);
+  return rc;
 }
 
 int gcctb_microbench(int32_t which, uint64_t n_words, uint64_t threads, uint64_t iters, double* rate, double* ms) {

@@ -213,6 +213,15 @@ int gcctb_write_txs_file(const char* path, const uint64_t* ops, const uint64_t* 
 /* Device information. */
 int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int32_t* cus);
 
+/* Stable device sort of n (key, value) pairs by the low key_bits bits of
+ * the key: the sort that the GaccO and GPUTx preparation runs where the
+ * reference calls thrust::sort_by_key / thrust::sort
+ * (include/gacco.cuh:307,331; include/gputx.cuh:252,266,281).  Keys are
+ * 32-bit (key_bytes 4) or 64-bit (key_bytes 8); keys and vals are host
+ * buffers, sorted in place; *ms (may be null) gets the device time of the
+ * sort alone.  Pairs with equal low key bits keep their input order. */
+int gcctb_sort_pairs(void* keys, uint32_t* vals, uint64_t n, int32_t key_bytes, int32_t key_bits, double* ms);
+
 /* Memory-system ceilings of the GPU for the roofline (no reference
  * counterpart; measured with the kernels' own access forms):
  *   which 0: random 64-bit agent-scope CAS over n_words words

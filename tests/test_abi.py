@@ -10,6 +10,7 @@ import pytest
 import gpu_cc_tb_amd as g
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EINVAL = -1  # GCCTB_EINVAL
 
 
 def declared_symbols():
@@ -65,3 +66,16 @@ def test_generator_rejects_bad_input():
             g.generate(100, 10, **args)
     with pytest.raises(g.GcctbError):
         g.generate(4, 10, theta=0.5, read_ratio=0.5, len_min=8)
+
+
+def test_sort_pairs_arguments():
+    """Argument checks run before any device call; an empty input is a no-op."""
+    import ctypes as C
+    lib = g.load_library()
+    k = np.zeros(4, dtype=np.uint32)
+    v = np.zeros(4, dtype=np.uint32)
+    kp, vp = k.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p)
+    assert lib.gcctb_sort_pairs(kp, vp, 4, 2, 8, None) == EINVAL
+    assert lib.gcctb_sort_pairs(kp, vp, 4, 4, 33, None) == EINVAL
+    assert lib.gcctb_sort_pairs(None, vp, 4, 4, 8, None) == EINVAL
+    assert lib.gcctb_sort_pairs(kp, vp, 0, 4, 8, None) == 0

@@ -68,6 +68,14 @@ def test_replay_parity(scheme, case):
         order[2::3] = np.arange(w.ntx)
         t3, r3, _ = orc.replay(rows, 10, seed, init, w.ops, w.tx_off, order)
         assert np.array_equal(res["table"], t3)
+        # same-rank conflicts: adjacent accesses of a row in (row, rank, op) order
+        # with equal rank where either writes (the identity index maps key to row)
+        txof = np.repeat(np.arange(w.ntx), np.diff(w.tx_off.astype(np.int64)))
+        row, rk, wr = g.op_key(w.ops).astype(np.int64), rank_o[txof], g.op_is_write(w.ops)
+        srt = np.lexsort((np.arange(row.size), rk, row))
+        row, rk, wr = row[srt], rk[srt], wr[srt]
+        same = (row[1:] == row[:-1]) & (rk[1:] == rk[:-1]) & (wr[1:] | wr[:-1])
+        assert st["same_rank_conflicts"] == int(same.sum())
     if cc is not None:
         if scheme in ("tpl_nw", "tpl_wd"):
             assert not np.any((cc >> np.uint64(1)) & np.uint64(0x7FFFFFFF)), "a lock is still held"
