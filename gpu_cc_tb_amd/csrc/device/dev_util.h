@@ -186,45 +186,31 @@ __device__ __forceinline__ uint64_t wave_fetch_add(uint64_t* ctr, uint32_t amoun
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 
-// ---- sorted-array index probe ---------------------------------------------
-// SortedArray_GPU::Find (include/index.cuh:74-91) returns the value stored for
-// `key`, or the index length if the key is absent.  On the device a miss is
+// Find (include/index.cuh:74-91) over the hash index: the key's bucket
+// holds it, or the lookup moves to the next bucket while every slot is taken.
+// Find returns the index length for an absent key; on the device a miss is
 // IDX_MISS instead, so that it can never be mistaken for a row when the index
-// is shorter than the table; gcctb_probe maps it back to the length.
-// The sorted (key,value)
-// array is stored in 64-byte leaves of 8 pairs; a static 16-ary directory of
-// separator keys (first key of each child) replaces the ~24 dependent probes
-// of a plain binary search with one 64-byte node per level.
-__device__ __forceinline__ uint32_t count_le16(const uint32_t* node, uint32_t key) {
-  const uint4* q = reinterpret_cast<const uint4*>(node);
-  uint32_t c = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint4 v = q[i];
-    c += (v.x <= key) + (v.y <= key) + (v.z <= key) + (v.w <= key);
-  }
-  return c;
-}
+// is shorter than the table (gcctb_probe maps it back to the length).
 __device__ __forceinline__ uint32_t index_find(const IndexView& ix, uint32_t key) {
-  // 0xffffffff pads unused directory slots, so it can never be a key; without
-  // this check it would count the padding and descend past the last child.
-  if (key == 0xffffffffu) return IDX_MISS;
-  uint32_t node = 0;
-  for (int l = (int)ix.nlevels - 1; l >= 0; --l) {
-    const uint32_t c = count_le16(ix.level[l] + (size_t)node * IDX_FANOUT, key);
-    if (c == 0) return IDX_MISS;
-    node = node * IDX_FANOUT + (c - 1);
-  }
-  const uint32_t* leaf = ix.leaves + (size_t)node * (2 * IDX_LEAF);
-  const uint4* q = reinterpret_cast<const uint4*>(leaf);
-  const uint4 k0 = q[0], k1 = q[1];
-  const uint32_t ks[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
-  int hit = -1;
+  if (key == IDX_EMPTY) return IDX_MISS;  // reserved, never a key
+  uint32_t b = idx_bucket(key, ix.nbuckets);
+  for (uint32_t step = 0; step < ix.nbuckets; ++step) {
+    const uint32_t* bk = ix.buckets + (size_t)b * (2 * IDX_SLOTS);
+    const uint4* q = reinterpret_cast<const uint4*>(bk);
+    const uint4 k0 = q[0], k1 = q[1];
+    const uint32_t ks[IDX_SLOTS] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+    int hit = -1;
+    bool free_slot = false;
 #pragma unroll
-  for (int i = 0; i < IDX_LEAF; ++i)
-    if (ks[i] == key) hit = i;
-  if (hit < 0) return IDX_MISS;
-  return leaf[IDX_LEAF + hit];
+    for (int i = 0; i < IDX_SLOTS; ++i) {
+      hit = ks[i] == key ? i : hit;
+      free_slot |= ks[i] == IDX_EMPTY;
+    }
+    if (hit >= 0) return bk[IDX_SLOTS + hit];
+    if (free_slot) return IDX_MISS;
+    b = b + 1 == ix.nbuckets ? 0u : b + 1;
+  }
+  return IDX_MISS;
 }
 
 }  // namespace dev

@@ -19,13 +19,11 @@ hipError_t exec_ordered(int G, const ExecArgs& a, uint64_t nslots, int threads, 
 
 // --- table / index ---
 hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s);
-// leaves from sorted keys/vals (device arrays); keys==nullptr: identity (key i -> value i)
-hipError_t build_leaves(uint32_t* leaves, const uint32_t* keys, const uint32_t* vals, uint64_t n,
-                        uint64_t nleaves, hipStream_t s);
-hipError_t build_level_from_leaves(uint32_t* dst, const uint32_t* leaves, uint64_t nleaves,
-                                   uint64_t padded, hipStream_t s);
-hipError_t build_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t nsrc_nodes,
-                                  uint64_t padded, hipStream_t s);
+// Insert n keys (keys == nullptr: key i) with values (vals == nullptr:
+// value i) into a hash index whose nbuckets buckets start all free
+// (every u32 IDX_EMPTY).  Keys must be unique and not IDX_EMPTY.
+hipError_t index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, uint32_t* buckets,
+                        uint32_t nbuckets, hipStream_t s);
 // rows_out[i] = row of ops[i]'s key; also into rows_copy when it is given
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
                  hipStream_t s, uint32_t* rows_copy = nullptr);

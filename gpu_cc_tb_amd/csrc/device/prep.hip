@@ -62,34 +62,28 @@ __global__ void k_seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint
   }
 }
 
-// leaf l holds keys[8l..8l+7] then vals[8l..8l+7]; missing entries get key
-// 0xffffffff (never a valid key) and value 0.
-__global__ void k_build_leaves(uint32_t* leaves, const uint32_t* keys, const uint32_t* vals,
-                               uint64_t n, uint64_t nleaves) {
-  const uint64_t total = nleaves * IDX_LEAF;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+// Hash index build: each key takes the first free slot along its bucket
+// sequence (CAS on the key word; a slot never frees again).  The table has
+// about twice the slots it needs, so the sequence is short; it is bounded by
+// nbuckets in any case.
+__global__ void k_index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, uint32_t* buckets,
+                               uint32_t nbuckets) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t l = i / IDX_LEAF, e = i % IDX_LEAF;
-    uint32_t k = 0xffffffffu, v = 0;
-    if (i < n) {
-      k = keys ? keys[i] : (uint32_t)i;
-      v = vals ? vals[i] : (uint32_t)i;
+    const uint32_t key = keys ? keys[i] : (uint32_t)i;
+    const uint32_t val = vals ? vals[i] : (uint32_t)i;
+    uint32_t b = idx_bucket(key, nbuckets);
+    for (uint32_t step = 0; step < nbuckets; ++step) {
+      uint32_t* bk = buckets + (size_t)b * (2 * IDX_SLOTS);
+      int s = 0;
+      while (s < IDX_SLOTS && atomicCAS(bk + s, IDX_EMPTY, key) != IDX_EMPTY) ++s;
+      if (s < IDX_SLOTS) {
+        bk[IDX_SLOTS + s] = val;
+        break;
+      }
+      b = b + 1 == nbuckets ? 0u : b + 1;
     }
-    leaves[l * 2 * IDX_LEAF + e] = k;
-    leaves[l * 2 * IDX_LEAF + IDX_LEAF + e] = v;
   }
-}
-__global__ void k_level_from_leaves(uint32_t* dst, const uint32_t* leaves, uint64_t nleaves,
-                                    uint64_t padded) {
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < padded;
-       j += (uint64_t)gridDim.x * blockDim.x)
-    dst[j] = j < nleaves ? leaves[j * 2 * IDX_LEAF] : 0xffffffffu;
-}
-__global__ void k_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t nsrc_nodes,
-                                   uint64_t padded) {
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < padded;
-       j += (uint64_t)gridDim.x * blockDim.x)
-    dst[j] = j < nsrc_nodes ? src[j * IDX_FANOUT] : 0xffffffffu;
 }
 
 // Quad (4 consecutive lanes) primitives over DPP quad permutes; every lane
@@ -110,58 +104,70 @@ __device__ __forceinline__ int quad_max(int v) {
   return o > v ? o : v;
 }
 
-// Index probe (Find, include/index.cuh:74-91, over the directory layout of
+// One bucket step of a quad-cooperative lookup: lane q holds the q-th 16
+// bytes of the bucket (keys 0-3, keys 4-7, values 0-3, values 4-7).  Sets
+// *val and returns true on a hit; *free_slot tells whether the bucket has a
+// free slot (then the key is absent).
+__device__ __forceinline__ bool quad_bucket_match(const uint4& w, uint32_t key, uint32_t q, uint32_t* val,
+                                                  bool* free_slot) {
+  const bool keys_here = q < 2;
+  int h = -1;
+  h = keys_here && w.x == key ? (int)q * 4 : h;
+  h = keys_here && w.y == key ? (int)q * 4 + 1 : h;
+  h = keys_here && w.z == key ? (int)q * 4 + 2 : h;
+  h = keys_here && w.w == key ? (int)q * 4 + 3 : h;
+  const int fr = keys_here && (w.x == IDX_EMPTY || w.y == IDX_EMPTY || w.z == IDX_EMPTY || w.w == IDX_EMPTY);
+  h = quad_max(h);
+  *free_slot = quad_max(fr) != 0;
+  // the value of slot h sits in lane 2 + h / 4, component h % 4
+  const uint32_t comp = (h & 3) == 0 ? w.x : (h & 3) == 1 ? w.y : (h & 3) == 2 ? w.z : w.w;
+  const uint32_t from2 = quad_get<2>(comp), from3 = quad_get<3>(comp);
+  *val = (h >> 2) ? from3 : from2;
+  return h >= 0;
+}
+
+// Index probe (Find, include/index.cuh:74-91, over the hash index of
 // dev::index_find).  A quad walks four probes at once: lane q loads op p0+q
-// (a wave loads 64 consecutive ops), the quad shares the four keys, and at
-// every level lane q reads the q-th 16 bytes of each probe's 64-byte node, so
-// a load instruction touches one line per probe instead of four and four
-// independent walks are in flight per quad.  Loads are unconditional (a
-// finished or missing probe reads node 0), so the four issue back to back.
+// (a wave loads 64 consecutive ops), the quad shares the four keys, and lane
+// q reads the q-th 16 bytes of each probe's home bucket, so one load
+// instruction covers one line per probe and the four buckets are read at
+// once.  A probe whose bucket is full without the key goes on bucket by
+// bucket, the quad together.
 __global__ __launch_bounds__(256) void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
                                                uint32_t* rows_copy) {
   const uint32_t q = threadIdx.x & 3u;
+  const uint4* bkt = reinterpret_cast<const uint4*>(ix.buckets);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of 4
   for (uint64_t p0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~3ull; p0 < n; p0 += stride) {
     const uint64_t mine = p0 + q;
-    const uint32_t mykey = mine < n ? op_key(ops[mine]) : 0xffffffffu;
+    const uint32_t mykey = mine < n ? op_key(ops[mine]) : IDX_EMPTY;
     const uint32_t key[4] = {quad_get<0>(mykey), quad_get<1>(mykey), quad_get<2>(mykey), quad_get<3>(mykey)};
-    uint32_t node[4] = {0u, 0u, 0u, 0u};
-    bool live[4];
+    uint32_t b[4];
+    uint4 w[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) live[j] = key[j] != 0xffffffffu;  // padding value: never a key
-    for (int l = (int)ix.nlevels - 1; l >= 0; --l) {
-      const uint4* lv = reinterpret_cast<const uint4*>(ix.level[l]);
-      uint4 v[4];
+    for (int j = 0; j < 4; ++j) b[j] = idx_bucket(key[j], ix.nbuckets);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = lv[(size_t)node[j] * 4 + q];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t c = quad_sum((v[j].x <= key[j]) + (v[j].y <= key[j]) + (v[j].z <= key[j]) +
-                                    (v[j].w <= key[j]));
-        if (live[j] && c == 0) live[j] = false;
-        node[j] = live[j] ? node[j] * IDX_FANOUT + (c - 1) : 0u;
-      }
-    }
-    // leaf: lanes 0,1 hold keys 0-3, 4-7; lanes 2,3 the values 0-3, 4-7
-    const uint4* lf = reinterpret_cast<const uint4*>(ix.leaves);
-    uint4 lv4[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) lv4[j] = lf[(size_t)node[j] * 4 + q];
+    for (int j = 0; j < 4; ++j) w[j] = bkt[(size_t)b[j] * 4 + q];
     uint32_t res[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const uint4 v = lv4[j];
-      // the last matching slot, as a scan of the leaf would find
-      const bool look = live[j] && q < 2;
-      int h = -1;
-      h = look && v.x == key[j] ? (int)q * 4 : h;
-      h = look && v.y == key[j] ? (int)q * 4 + 1 : h;
-      h = look && v.z == key[j] ? (int)q * 4 + 2 : h;
-      h = look && v.w == key[j] ? (int)q * 4 + 3 : h;
-      h = quad_max(h);
-      const uint32_t comp = (h & 3) == 0 ? v.x : (h & 3) == 1 ? v.y : (h & 3) == 2 ? v.z : v.w;
-      const uint32_t from2 = quad_get<2>(comp), from3 = quad_get<3>(comp);
-      res[j] = h < 0 ? IDX_MISS : (h >> 2) ? from3 : from2;
+      uint32_t r = IDX_MISS;
+      if (key[j] != IDX_EMPTY) {  // the reserved key is never stored
+        uint4 cur = w[j];
+        uint32_t bb = b[j];
+        for (uint32_t step = 1;; ++step) {
+          uint32_t val;
+          bool free_slot;
+          if (quad_bucket_match(cur, key[j], q, &val, &free_slot)) {
+            r = val;
+            break;
+          }
+          if (free_slot || step >= ix.nbuckets) break;
+          bb = bb + 1 == ix.nbuckets ? 0u : bb + 1;
+          cur = bkt[(size_t)bb * 4 + q];
+        }
+      }
+      res[j] = r;
     }
     const uint32_t r = q == 0 ? res[0] : q == 1 ? res[1] : q == 2 ? res[2] : res[3];
     if (mine < n) {
@@ -627,17 +633,9 @@ namespace launch {
 hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s) {
   GC_LAUNCH(k_seed_table, rows * cols, cells, rows, cols, seed);
 }
-hipError_t build_leaves(uint32_t* leaves, const uint32_t* keys, const uint32_t* vals, uint64_t n,
-                        uint64_t nleaves, hipStream_t s) {
-  GC_LAUNCH(k_build_leaves, nleaves * IDX_LEAF, leaves, keys, vals, n, nleaves);
-}
-hipError_t build_level_from_leaves(uint32_t* dst, const uint32_t* leaves, uint64_t nleaves,
-                                   uint64_t padded, hipStream_t s) {
-  GC_LAUNCH(k_level_from_leaves, padded, dst, leaves, nleaves, padded);
-}
-hipError_t build_level_from_level(uint32_t* dst, const uint32_t* src, uint64_t nsrc_nodes,
-                                  uint64_t padded, hipStream_t s) {
-  GC_LAUNCH(k_level_from_level, padded, dst, src, nsrc_nodes, padded);
+hipError_t index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, uint32_t* buckets,
+                        uint32_t nbuckets, hipStream_t s) {
+  GC_LAUNCH(k_index_insert, n, keys, vals, n, buckets, nbuckets);
 }
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
                  hipStream_t s, uint32_t* rows_copy) {

@@ -193,19 +193,31 @@ constexpr uint64_t ERR_TXLEN = 8;           // transaction longer than the kerne
 constexpr uint64_t ERR_WATCHDOG = 16;       // a wave exceeded the watchdog time and gave up
 
 // ---------------------------------------------------------------------------
-// Sorted-array index view (keys sorted ascending, leaves of 8 (key,val) pairs,
-// internal directory levels of 16 separators per 64-byte node).
+// Key -> row index with Find's contract (include/index.cuh:74-91): a key's
+// value, or the index length on a miss.  Keys are unique (the build rejects
+// duplicates), so a hash table answers exactly what the sorted array does, in
+// one 64-byte bucket read instead of a search.  nbuckets buckets of
+// [8 keys][8 values]; IDX_EMPTY (0xffffffff, a reserved key) marks a free
+// slot.  A key sits in the first bucket that had a free slot when it was
+// inserted, along idx_bucket(key), +1, +2, ... (mod nbuckets); slots never
+// empty again, so a lookup stops at the key or at a bucket with a free slot.
 // ---------------------------------------------------------------------------
-constexpr uint32_t IDX_MISS = 0xffffffffu;  // device-side probe result for an absent key
-constexpr int IDX_LEAF = 8;
-constexpr int IDX_FANOUT = 16;
-constexpr int IDX_MAX_LEVELS = 12;
+constexpr uint32_t IDX_MISS = 0xffffffffu;   // device-side probe result for an absent key
+constexpr uint32_t IDX_EMPTY = 0xffffffffu;  // free slot
+constexpr int IDX_SLOTS = 8;                 // keys per bucket
 struct IndexView {
-  const uint32_t* leaves;                 // nleaves * 16 u32: [8 keys][8 vals]
-  const uint32_t* level[IDX_MAX_LEVELS];  // level[0] = separators over leaves
-  uint32_t nlevels;                       // internal levels; level[nlevels-1] is one node
-  uint32_t length;                        // number of keys; Find miss returns length
+  const uint32_t* buckets;  // nbuckets * 16 u32: [8 keys][8 values]
+  uint32_t nbuckets;
+  uint32_t length;          // number of keys; Find's miss value
 };
+// Home bucket: Fibonacci hashing (multiply by 2^32 / golden ratio, an odd
+// constant, so distinct keys keep distinct products), mapped onto
+// [0, nbuckets) by the high half of a 32 x 32 product.
+GC_HD uint32_t idx_bucket(uint32_t key, uint32_t nbuckets) {
+  return (uint32_t)(((uint64_t)(uint32_t)(key * 0x9E3779B1u) * nbuckets) >> 32);
+}
+// Buckets for n keys: about half the slots stay free.
+GC_HD uint64_t idx_buckets_for(uint64_t n) { return n / 4 + 1; }
 
 // ---------------------------------------------------------------------------
 // Kernel argument block for the execution kernels.

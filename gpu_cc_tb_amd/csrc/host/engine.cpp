@@ -88,8 +88,7 @@ struct gcctb_ctx {
   hipEvent_t ev[8] = {};
   // table + index
   gcctb::DevBuf<uint64_t> cells;
-  gcctb::DevBuf<uint32_t> leaves;
-  gcctb::DevBuf<uint32_t> levels[gcctb::IDX_MAX_LEVELS];
+  gcctb::DevBuf<uint32_t> buckets;  // hash index
   gcctb::IndexView ix{};
   bool table_ready = false;
   // transactions
@@ -396,39 +395,19 @@ int hip_fail(const HipFailure& f) {
   return set_error(GCCTB_EDEVICE, std::string("HIP error: ") + hipGetErrorString(f.err) + " in " + f.where);
 }
 
-void build_index_levels(gcctb_ctx& c, uint64_t n) {
-  const uint64_t nleaves = (n + IDX_LEAF - 1) / IDX_LEAF;
-  uint64_t entries = nleaves;
-  int l = 0;
-  while (true) {
-    if (l >= IDX_MAX_LEVELS) throw HipFailure{hipErrorInvalidValue, "index too deep"};
-    const uint64_t nodes = (entries + IDX_FANOUT - 1) / IDX_FANOUT;
-    const uint64_t padded = nodes * IDX_FANOUT;
-    c.levels[l].alloc(padded);
-    if (l == 0)
-      HIP_TRY(launch::build_level_from_leaves(c.levels[0].p, c.leaves.p, nleaves, padded, c.stream));
-    else
-      HIP_TRY(launch::build_level_from_level(c.levels[l].p, c.levels[l - 1].p, entries, padded, c.stream));
-    c.ix.level[l] = c.levels[l].p;
-    ++l;
-    if (nodes == 1) break;
-    entries = nodes;
-  }
-  for (int k = l; k < IDX_MAX_LEVELS; ++k) {
-    c.levels[k].release();
-    c.ix.level[k] = nullptr;
-  }
-  c.ix.leaves = c.leaves.p;
-  c.ix.nlevels = (uint32_t)l;
-  c.ix.length = (uint32_t)n;
-}
-
+// Hash index over n keys (dkeys / dvals device arrays; nullptr: identity).
 void build_index(gcctb_ctx& c, const uint32_t* dkeys, const uint32_t* dvals, uint64_t n) {
-  const uint64_t nleaves = (n + IDX_LEAF - 1) / IDX_LEAF;
-  c.leaves.alloc(nleaves * 2 * IDX_LEAF);
-  HIP_TRY(launch::build_leaves(c.leaves.p, dkeys, dvals, n, nleaves, c.stream));
-  build_index_levels(c, n);
+  const uint64_t nb = idx_buckets_for(n);
+  if (nb > 0xffffffffull) throw HipFailure{hipErrorInvalidValue, "index too large"};
+  c.ix = IndexView{};
+  c.buckets.release();
+  c.buckets.alloc(nb * 2 * IDX_SLOTS);
+  HIP_TRY(launch::fill_u32(c.buckets.p, nb * 2 * IDX_SLOTS, IDX_EMPTY, c.stream));
+  HIP_TRY(launch::index_insert(dkeys, dvals, n, c.buckets.p, (uint32_t)nb, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
+  c.ix.buckets = c.buckets.p;
+  c.ix.nbuckets = (uint32_t)nb;
+  c.ix.length = (uint32_t)n;
 }
 
 void ensure_state(gcctb_ctx& c) {
@@ -575,7 +554,7 @@ int gcctb_build_index(gcctb_ctx* c, const uint32_t* keys, const uint32_t* vals, 
 
 int gcctb_probe(gcctb_ctx* c, const uint32_t* keys, uint64_t n, uint32_t* rows_out) {
   if (!c || (n && (!keys || !rows_out))) return set_error(GCCTB_EINVAL, "null argument");
-  if (!c->ix.leaves) return set_error(GCCTB_ESTATE, "no index");
+  if (!c->ix.buckets) return set_error(GCCTB_ESTATE, "no index");
   if (n == 0) return GCCTB_OK;
   try {
     std::vector<uint64_t> ops(n);
@@ -603,7 +582,7 @@ int gcctb_load_table(gcctb_ctx* c, const uint64_t* cells, uint64_t rows, uint32_
   try {
     HIP_TRY(hipMemcpyAsync(c->cells.p, cells, rows * cols * 8, hipMemcpyHostToDevice, c->stream));
     ensure_state(*c);
-    if (!c->ix.leaves) build_index(*c, nullptr, nullptr, rows);
+    if (!c->ix.buckets) build_index(*c, nullptr, nullptr, rows);
     HIP_TRY(hipStreamSynchronize(c->stream));
   } catch (const HipFailure& f) {
     return hip_fail(f);
@@ -617,7 +596,7 @@ int gcctb_seed_table(gcctb_ctx* c) {
   try {
     HIP_TRY(launch::seed_table(c->cells.p, c->cfg.rows, c->cfg.cols, c->cfg.seed, c->stream));
     ensure_state(*c);
-    if (!c->ix.leaves) build_index(*c, nullptr, nullptr, c->cfg.rows);
+    if (!c->ix.buckets) build_index(*c, nullptr, nullptr, c->cfg.rows);
     HIP_TRY(hipStreamSynchronize(c->stream));
   } catch (const HipFailure& f) {
     return hip_fail(f);

@@ -14,6 +14,28 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "gpu_cc_tb_amd", "bin", "test_ycsb")
 
 
+def idx_bucket(keys, nb):
+    """Home bucket of each key in an index of nb buckets (Fibonacci hashing,
+    then the high half of a 32 x 32 product)."""
+    h = (np.asarray(keys, dtype=np.uint64) * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
+    return ((h * np.uint64(nb)) >> np.uint64(32)).astype(np.int64)
+
+
+def clustered_keys(n, ncluster, seed):
+    """n distinct keys, ncluster of them in the index's last home bucket: its
+    probe chain runs past the end and wraps to bucket 0.  Also returns more
+    keys of that bucket that are not in the index."""
+    nb = n // 4 + 1  # buckets of an index over n keys
+    cand = np.arange(1, 1 << 22, dtype=np.uint64)
+    home = idx_bucket(cand, nb)
+    last, rest = cand[home == nb - 1], cand[home != nb - 1]
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(last.size, ncluster + 50, replace=False)
+    keys = np.concatenate([last[pick[:ncluster]], rng.choice(rest, n - ncluster, replace=False)])
+    rng.shuffle(keys)
+    return keys.astype(np.uint32), last[pick[ncluster:]].astype(np.uint32)
+
+
 def random_index(rows, n, seed):
     r = np.random.default_rng(seed)
     keys = r.choice(np.uint64(0xFFFFFFFE), size=n, replace=False).astype(np.uint32)
@@ -41,17 +63,40 @@ def test_probe_custom_index():
 
 
 @pytest.mark.gpu
+def test_probe_long_chains():
+    """Keys crowded into one home bucket: hits along a chain that wraps past
+    the last bucket, and misses that walk the chain to a free slot."""
+    n = 400
+    keys, absent = clustered_keys(n, 120, 5)
+    vals = np.random.default_rng(6).permutation(n).astype(np.uint32)
+    probes = np.concatenate([keys, absent, np.array([0, 0xFFFFFFFE], dtype=np.uint32)])
+    print(f"[chains] n={n} probes={probes.size}", flush=True)
+    with g.Context("tpl_nw", rows=n) as ctx:
+        ctx.seed_table()
+        ctx.build_index(keys, vals)
+        got = ctx.probe(probes).astype(np.int64)
+    exp = np.concatenate([vals.astype(np.int64), np.full(absent.size, n),
+                          [n if 0 not in keys else int(vals[np.nonzero(keys == 0)[0][0]]), n]])
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("form", ["plain", "persistent"])
+@pytest.mark.parametrize("keyset", ["random", "chains"])
 @pytest.mark.parametrize("scheme", ["silo", "mvcc", "tpl_wd", "gacco", "gputx"])
-def test_run_through_custom_index(scheme, form):
+def test_run_through_custom_index(scheme, keyset, form):
     """Transactions whose keys are arbitrary 32-bit values, resolved to rows
-    through the index: the result equals the run on row keys."""
+    through the index (inside the execution kernel in the plain form, by the
+    probe pass in the persistent form): the result equals the run on row keys."""
     rows, ntx = 2048, 1500
     w = g.generate(rows, ntx, theta=0.8, read_ratio=0.7, len_min=8, len_max=16)
-    keys = np.random.default_rng(21).choice(np.uint64(0xFFFFFFFE), size=rows, replace=False).astype(np.uint32)
+    if keyset == "random":
+        keys = np.random.default_rng(21).choice(np.uint64(0xFFFFFFFE), size=rows, replace=False).astype(np.uint32)
+    else:
+        keys, _ = clustered_keys(rows, 300, 22)
     row_of_op = g.op_key(w.ops).astype(np.int64)
     mapped = g.Ops(g.make_ops(keys[row_of_op], g.op_col(w.ops), g.op_is_write(w.ops)), w.tx_off)
-    print(f"[index-run] scheme={scheme} form={form}", flush=True)
+    print(f"[index-run] scheme={scheme} keyset={keyset} form={form}", flush=True)
     with g.Context(scheme, rows=rows, warp_density=3, block_size=8, batch_size=500) as ctx:
         ctx.seed_table()
         ctx.build_index(keys, np.arange(rows, dtype=np.uint32))
