@@ -69,6 +69,8 @@ hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t
 // writes, from the (row << rank_bits | rank) sorted keys and the group flags.
 hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
                          hipStream_t s);
+hipError_t gtx_conflicts32(const uint32_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
+                           hipStream_t s);
 // GPUTx one-pass rank (src/gputx.cu:5-27).  boundary[p] = 1 if p is not the
 // first access of its row and (p-1 or p) is a write; rowstart[p] = p at row
 // starts else 0.
@@ -80,7 +82,12 @@ hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, 
 // composite key (row << rank_bits) | rank[op_tx[op]] for ops [0, n)
 hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                          uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s);
+// the same key cut to 32 bits, for row_bits + rank_bits <= 32
+hipError_t gtx_exec_keys32(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                           uint64_t n, int rank_bits, uint32_t* keys, hipStream_t s);
+// out[i] = keys[i] >> shift
 hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s);
+hipError_t u32_key_high(const uint32_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s);
 hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks, uint32_t* hist, hipStream_t s);
 
 // --- memory-system ceilings (microbench.hip) ---

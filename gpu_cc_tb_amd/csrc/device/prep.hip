@@ -505,7 +505,8 @@ __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const u
 // where either writes.  In the (row << rank_bits | rank) sorted keys, same
 // row and rank means equal keys; flags[p] (access_group_flags) is non-zero
 // at a non-row-start position exactly when either access writes.
-__global__ void k_gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n,
+template <class KT>
+__global__ void k_gtx_conflicts(const KT* skey, const uint32_t* flags, uint64_t n,
                                 unsigned long long* conflicts) {
   uint64_t local = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; p < n;
@@ -540,13 +541,15 @@ __global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_
   mx = (uint32_t)block_reduce_u64<true>(mx);
   if (threadIdx.x == 0 && mx) atomicMax(max_rank, mx);
 }
+template <class KT>
 __global__ void k_gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
-                                uint64_t n, int rank_bits, uint64_t* keys) {
+                                uint64_t n, int rank_bits, KT* keys) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
-    keys[i] = ((uint64_t)rows[i] << rank_bits) | rank[op_tx[i]];
+    keys[i] = (KT)(((uint64_t)rows[i] << rank_bits) | rank[op_tx[i]]);
 }
-__global__ void k_u64_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out) {
+template <class KT>
+__global__ void k_key_high(const KT* keys, uint64_t n, int shift, uint32_t* out) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = (uint32_t)(keys[i] >> shift);
@@ -688,7 +691,11 @@ hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t
 }
 hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
                          hipStream_t s) {
-  GC_LAUNCH_REDUCE(k_gtx_conflicts, n, skey, flags, n, (unsigned long long*)conflicts);
+  GC_LAUNCH_REDUCE(k_gtx_conflicts<uint64_t>, n, skey, flags, n, (unsigned long long*)conflicts);
+}
+hipError_t gtx_conflicts32(const uint32_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
+                           hipStream_t s) {
+  GC_LAUNCH_REDUCE(k_gtx_conflicts<uint32_t>, n, skey, flags, n, (unsigned long long*)conflicts);
 }
 hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint64_t nrows, uint32_t* boundary,
                      uint32_t* rowstart, hipStream_t s) {
@@ -700,10 +707,17 @@ hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, 
 }
 hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                          uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s) {
-  GC_LAUNCH(k_gtx_exec_keys, n, rows, op_tx, rank, n, rank_bits, keys);
+  GC_LAUNCH(k_gtx_exec_keys<uint64_t>, n, rows, op_tx, rank, n, rank_bits, keys);
+}
+hipError_t gtx_exec_keys32(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                           uint64_t n, int rank_bits, uint32_t* keys, hipStream_t s) {
+  GC_LAUNCH(k_gtx_exec_keys<uint32_t>, n, rows, op_tx, rank, n, rank_bits, keys);
 }
 hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s) {
-  GC_LAUNCH(k_u64_high, n, keys, n, shift, out);
+  GC_LAUNCH(k_key_high<uint64_t>, n, keys, n, shift, out);
+}
+hipError_t u32_key_high(const uint32_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s) {
+  GC_LAUNCH(k_key_high<uint32_t>, n, keys, n, shift, out);
 }
 hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks, uint32_t* hist, hipStream_t s) {
   uint64_t blocks = (ntx + 4095) / 4096;

@@ -293,7 +293,8 @@ class GaccoHost : public SchemeHost {
     HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.rows_out.p + o0, s, ctx.sk0.p));
     HIP_TRY(launch::op_ids(ctx.ops.p, n, (uint32_t)o0, ctx.sv0.p, s));
     uint32_t *srow, *sop;
-    HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.ix.length),
+    // rows are below cfg.rows; a miss (IDX_MISS) has all row bits set and sorts last
+    HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.cfg.rows),
                                    ctx.sort_tmp.p, &srow, &sop, s));
     HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s));
     HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
@@ -320,7 +321,8 @@ class GPUTxHost : public SchemeHost {
     HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p));
     HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
     uint32_t *srow, *sop;
-    const int row_bits = bits_for(ctx.ix.length);
+    // rows are below cfg.rows; a miss (IDX_MISS) has all row bits set and sorts last
+    const int row_bits = bits_for(ctx.cfg.rows);
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, row_bits, ctx.sort_tmp.p,
                                    &srow, &sop, s));
     // one-pass local ranks and rank[tx] = max (src/gputx.cu:5-27)
@@ -357,8 +359,20 @@ class GPUTxHost : public SchemeHost {
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, ntx, rank_bits, ctx.sort_tmp.p,
                                    &srank, &stx, s));
     HIP_TRY(hipMemcpyAsync(ctx.tx_map.p, stx, ntx * 4, hipMemcpyDeviceToDevice, s));
-    // execution order of every row: (row, rank, tx)
-    if (n) {
+    // execution order of every row: (row, rank, tx); 32-bit composite keys
+    // when row and rank fit, else 64-bit
+    if (n && row_bits + rank_bits <= 32) {
+      HIP_TRY(launch::gtx_exec_keys32(ctx.rows_out.p, ctx.op_tx.p, ctx.rank.p, n, rank_bits, ctx.sk0.p, s));
+      HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
+      uint32_t *skey, *sop2;
+      HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, row_bits + rank_bits,
+                                     ctx.sort_tmp.p, &skey, &sop2, s));
+      uint32_t* srow2 = ctx.S.p;  // free once the ranks are made
+      HIP_TRY(launch::u32_key_high(skey, n, rank_bits, srow2, s));
+      ordered_need(srow2, sop2);
+      HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
+      HIP_TRY(launch::gtx_conflicts32(skey, ctx.flags.p, n, ctx.conflicts.p, s));
+    } else if (n) {
       HIP_TRY(launch::gtx_exec_keys(ctx.rows_out.p, ctx.op_tx.p, ctx.rank.p, n, rank_bits, ctx.sk64a.p, s));
       HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
       uint64_t* skey;
@@ -366,13 +380,19 @@ class GPUTxHost : public SchemeHost {
       HIP_TRY(launch::radix_sort_u64(ctx.sk64a.p, ctx.sv0.p, ctx.sk64b.p, ctx.sv1.p, n, row_bits + rank_bits,
                                      ctx.sort_tmp.p, &skey, &sop2, s));
       HIP_TRY(launch::u64_key_high(skey, n, rank_bits, ctx.sk0.p, s));
-      HIP_TRY(launch::access_group_flags(ctx.sk0.p, sop2, n, ctx.flags.p, s));
-      HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
-      HIP_TRY(launch::access_need(ctx.sk0.p, sop2, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p,
-                                  ctx.cfg.rows, s));
+      ordered_need(ctx.sk0.p, sop2);
       HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
       HIP_TRY(launch::gtx_conflicts(skey, ctx.flags.p, n, ctx.conflicts.p, s));
     }
+  }
+  // need[] and counters[] from accesses in execution order (group flags in
+  // ctx.flags, which the conflict count reads afterwards)
+  void ordered_need(const uint32_t* srow, const uint32_t* sop) {
+    const uint64_t n = ctx.nops;
+    hipStream_t s = ctx.stream;
+    HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s));
+    HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
+    HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows, s));
   }
   void Init(int, uint64_t) override {}
   // the persistent form runs every rank level in one launch

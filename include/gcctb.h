@@ -85,9 +85,11 @@ int gcctb_load_table(gcctb_ctx* ctx, const uint64_t* cells, uint64_t rows, uint3
 /* Seed the table on the device: cell[r][c] = splitmix64(seed ^ (r*cols + c)). */
 int gcctb_seed_table(gcctb_ctx* ctx);
 
-/* Build the sorted-array index over the id column (key i -> row i):
- * replaces Table_CPU::AddIndex(SORTED_ARRAY, 0) (include/table.cuh:95-130,
- * include/index.cuh:96-111).  Called by load_table/seed_table. */
+/* Build the index over the id column (key i -> row i): replaces
+ * Table_CPU::AddIndex(SORTED_ARRAY, 0) (include/table.cuh:95-130,
+ * include/index.cuh:96-111).  The index answers Find's question (value, or
+ * the length on a miss) from a hash table of unique keys instead of a sorted
+ * array.  Called by load_table/seed_table. */
 int gcctb_build_index_identity(gcctb_ctx* ctx);
 /* Build the index from host (key, value) pairs (any order; keys distinct,
  * keys < 2^32-1, values < rows). */

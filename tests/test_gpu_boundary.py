@@ -82,32 +82,42 @@ def test_probe_long_chains():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form", ["plain", "persistent"])
-@pytest.mark.parametrize("keyset", ["random", "chains"])
+@pytest.mark.parametrize("keyset", ["random", "chains", "short"])
 @pytest.mark.parametrize("scheme", ["silo", "mvcc", "tpl_wd", "gacco", "gputx"])
 def test_run_through_custom_index(scheme, keyset, form):
     """Transactions whose keys are arbitrary 32-bit values, resolved to rows
     through the index (inside the execution kernel in the plain form, by the
-    probe pass in the persistent form): the result equals the run on row keys."""
+    probe pass in the persistent form): the result equals the serial replay on
+    the rows the index maps them to.  "short": an index of fewer keys than
+    the table has rows, whose values reach the last rows."""
     rows, ntx = 2048, 1500
-    w = g.generate(rows, ntx, theta=0.8, read_ratio=0.7, len_min=8, len_max=16)
-    if keyset == "random":
-        keys = np.random.default_rng(21).choice(np.uint64(0xFFFFFFFE), size=rows, replace=False).astype(np.uint32)
+    nkeys = 700 if keyset == "short" else rows
+    w = g.generate(nkeys, ntx, theta=0.8, read_ratio=0.7, len_min=8, len_max=16)
+    rng = np.random.default_rng(21)
+    if keyset == "chains":
+        keys, _ = clustered_keys(nkeys, 300, 22)
     else:
-        keys, _ = clustered_keys(rows, 300, 22)
-    row_of_op = g.op_key(w.ops).astype(np.int64)
-    mapped = g.Ops(g.make_ops(keys[row_of_op], g.op_col(w.ops), g.op_is_write(w.ops)), w.tx_off)
+        keys = rng.choice(np.uint64(0xFFFFFFFE), size=nkeys, replace=False).astype(np.uint32)
+    vals = np.sort(rng.choice(rows, nkeys, replace=False)) if keyset == "short" else np.arange(rows)
+    vals = vals.astype(np.uint32)
+    if keyset == "short":
+        assert vals.max() >= 2048 - 64 and nkeys < 1024  # rows past the index length's bit width
+    ent = g.op_key(w.ops).astype(np.int64)  # index entry of each op
+    col, isw = g.op_col(w.ops), g.op_is_write(w.ops)
+    mapped = g.Ops(g.make_ops(keys[ent], col, isw), w.tx_off)
+    on_rows = g.make_ops(vals[ent], col, isw)  # the same ops addressed by table row
     print(f"[index-run] scheme={scheme} keyset={keyset} form={form}", flush=True)
     with g.Context(scheme, rows=rows, warp_density=3, block_size=8, batch_size=500) as ctx:
         ctx.seed_table()
-        ctx.build_index(keys, np.arange(rows, dtype=np.uint32))
+        ctx.build_index(keys, vals)
         ctx.load_ops(mapped)
         st = ctx.run(form)
         res = ctx.fetch()
     assert st["committed"] == ntx and st["index_miss"] == 0
     init = orc.seed_table(g.SEED, rows, 10)
-    exp_t, exp_r, ties = orc.replay(rows, 10, g.SEED, init, w.ops, w.tx_off, res["order"])
+    exp_t, exp_r, ties = orc.replay(rows, 10, g.SEED, init, on_rows, w.tx_off, res["order"])
     assert ties == 0
-    isr = ~g.op_is_write(w.ops)
+    isr = ~isw
     assert np.array_equal(res["reads"][isr], exp_r[isr])
     assert np.array_equal(res["table"], exp_t)
 

@@ -143,3 +143,32 @@ def test_persistent_urgent_drain(scheme, monkeypatch):
     assert st["committed"] == ntx and st["error_flags"] == 0
     assert int(res["aborts"].max()) >= 16, "the stream should drive some transaction into the urgent state"
     check_replay(w, res, rows, g.SEED)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["plain", "persistent"])
+def test_gputx_wide_order_keys(form):
+    """GPUTx whose (row, rank) execution keys need more than 32 bits (2^20
+    rows, over 4,096 rank levels): the 64-bit key path, against the oracle's
+    ranks and the (rank, id) replay."""
+    rows, ntx = 1 << 20, 6000
+    rng = np.random.default_rng(31)
+    # every transaction writes row 7 (one rank level each) and reads a row
+    # from the top of the table (row bits 20)
+    hot = g.make_ops(np.full(ntx, 7), rng.integers(0, 10, ntx), np.ones(ntx, dtype=bool))
+    far = g.make_ops(rng.integers(rows - 4096, rows, ntx), rng.integers(0, 10, ntx), np.zeros(ntx, dtype=bool))
+    ops = np.empty(2 * ntx, dtype=np.uint64)
+    ops[0::2], ops[1::2] = hot, far
+    w = g.Ops(ops, np.arange(0, 2 * ntx + 1, 2, dtype=np.uint64))
+    ranks, _ = orc.gputx_rank(rows, w.ops, w.tx_off)
+    nranks = int(ranks.max()) + 1
+    print(f"[gputx-wide] form={form} ranks={nranks}", flush=True)
+    row_bits, rank_bits = int(rows).bit_length(), int(nranks - 1).bit_length()
+    assert nranks > 4096 and row_bits + rank_bits > 32
+    with g.Context("gputx", rows=rows, warp_density=1, block_size=8, batch_size=ntx) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run(form)
+        res = ctx.fetch()
+    assert st["committed"] == ntx and st["ranks"] == nranks
+    check_replay(w, res, rows, g.SEED)
