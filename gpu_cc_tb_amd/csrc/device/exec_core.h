@@ -77,7 +77,8 @@ __device__ __forceinline__ uint32_t op_index(int k) {
 // transactions of one wave that restart in the same round collide again in the
 // same order; an immediate retry (benchmark/ycsb.cu:99-104) can then repeat
 // forever.  The restart is delayed by a random number of units in
-// [0, 2^min(attempts, 10)), where a unit is the duration of the failed attempt
+// [0, 2^min(attempts, backoff_exp)) (backoff_exp 10 unless configured), where
+// a unit is the duration of the failed attempt
 // (at least backoff_ticks, at most BACKOFF_UNIT_MAX): retries spread over
 // rounds, not just over ticks.  The cap matters for attempts that waited
 // (wait-die, T/O reads of uncommitted rows): their duration includes the
@@ -108,7 +109,7 @@ __device__ __forceinline__ void abort_attempt(TxCtx<G>& c, const ExecArgs& a, ui
   uint64_t unit = dur > a.backoff_ticks ? dur : a.backoff_ticks;
   if (unit > BACKOFF_UNIT_MAX) unit = BACKOFF_UNIT_MAX;
   const uint64_t h = hash3(a.seed ^ TAG_BACKOFF, c.tx, c.attempts);
-  const uint32_t span = 1u << (c.attempts < 10 ? c.attempts : 10);
+  const uint32_t span = 1u << (c.attempts < a.backoff_exp ? c.attempts : a.backoff_exp);
   c.t_resume = t + (h % span) * unit;
   c.ph = PH_START;
 }

@@ -263,6 +263,7 @@ struct ExecArgs {
   uint32_t adapt;          // persistent form: adapt the window to the abort rate
   uint32_t persist;        // 1 in persistent launches (slots from CTR_SLOT, gates, urgent state)
   uint32_t urgent_attempts; // persistent form: aborts that make a transaction urgent (0 = never)
+  uint32_t backoff_exp;     // retry delay spans at most 2^backoff_exp units
   uint64_t* tcommit;       // optional, per tx: wall-clock tick of its commit (serving form)
 };
 

@@ -84,6 +84,7 @@ struct gcctb_ctx {
   int threads = 1024;
   uint64_t watchdog_ticks = 0;
   uint32_t urgent_attempts = 256;
+  uint32_t backoff_exp = 10;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   // table + index
@@ -219,6 +220,7 @@ class SchemeHost {
     a.tx_rank = nullptr;
     a.watchdog_ticks = ctx.watchdog_ticks;
     a.urgent_attempts = ctx.urgent_attempts;
+    a.backoff_exp = ctx.backoff_exp;
     a.dbg = ctx.dbg.p;
     a.persist_groups = ctx.cfg.persist_window;
     a.adapt = ctx.cfg.persist_adaptive ? 1u : 0u;
@@ -491,6 +493,11 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
     // persistent form: aborts after which a transaction pauses admission
     // until it commits (GCCTB_URGENT_ATTEMPTS, 0 = never)
     if (const char* ua = std::getenv("GCCTB_URGENT_ATTEMPTS")) c->urgent_attempts = (uint32_t)std::atoi(ua);
+    // retry delay: at most 2^GCCTB_BACKOFF_EXP units (0..16, default 10)
+    if (const char* be = std::getenv("GCCTB_BACKOFF_EXP")) {
+      const int e = std::atoi(be);
+      c->backoff_exp = (uint32_t)(e < 0 ? 0 : e > 16 ? 16 : e);
+    }
     const char* dump = std::getenv("GCCTB_DEBUG_DUMP");
     if (dump && *dump) c->dbg_path = dump;
   }
