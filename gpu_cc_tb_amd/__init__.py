@@ -221,7 +221,8 @@ def device_info() -> dict:
     return {"name": name.value.decode(), "wall_clock_khz": khz.value, "cus": cus.value}
 
 
-MICROBENCH = {"cas_random": 0, "add_random": 1, "add_same": 2, "load_coherent": 3, "load_cached": 4, "copy": 5}
+MICROBENCH = {"cas_random": 0, "add_random": 1, "add_same": 2, "load_coherent": 3, "load_cached": 4, "copy": 5,
+              "load_pair_one_line": 6, "load_pair_two_lines": 7, "ts_read_one_line": 8, "ts_read_two_lines": 9}
 
 
 def microbench(kind: str, n_words: int, threads: int = 1 << 20, iters: int = 16) -> dict:

@@ -49,6 +49,38 @@ __global__ __launch_bounds__(256) void mb_load_random(const uint64_t* words, uin
   }
   if (acc == 0x5eed) atomicAdd(sink, 1ull);
 }
+// Pairs of coherent 8-byte loads issued back to back: in one 128-byte line
+// (offsets 0 and 80, like a row's cell and a word stored beside it) or in two
+// random lines.  Their ratio says whether the second access of a line is free.
+__global__ __launch_bounds__(256) void mb_load_pairs(const uint64_t* words, uint64_t n, uint64_t iters, uint64_t seed,
+                                                      int same_line, unsigned long long* sink) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t lines = n / 16;
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < iters; ++i) {
+    const uint64_t r = rnd(seed, t * iters + i);
+    const uint64_t* p = words + (r % lines) * 16;
+    const uint64_t* q = same_line ? p + 10 : words + (rnd(seed ^ 0x51, t * iters + i) % lines) * 16 + 10;
+    acc += dev::ld(p) + dev::ld(q);
+  }
+  if (acc == 0x5eed) atomicAdd(sink, 1ull);
+}
+// A timestamp read: load a word, load a cell, CAS the word, with the cell in
+// the word's 128-byte line or in another line.
+__global__ __launch_bounds__(256) void mb_ts_read(uint64_t* words, uint64_t n, uint64_t iters, uint64_t seed,
+                                                   int same_line, unsigned long long* sink) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t lines = n / 16;
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < iters; ++i) {
+    uint64_t* w = words + (rnd(seed, t * iters + i) % lines) * 16 + 15;
+    const uint64_t* c = same_line ? w - 15 : words + (rnd(seed ^ 0x51, t * iters + i) % lines) * 16;
+    uint64_t cur = dev::ld(w);
+    acc += dev::ld(c);
+    dev::cas(w, cur, cur + 1);
+  }
+  if (acc == 0x5eed) atomicAdd(sink, 1ull);
+}
 __global__ __launch_bounds__(256) void mb_copy(const uint4* src, uint4* dst, uint64_t n16) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
@@ -59,8 +91,11 @@ __global__ __launch_bounds__(256) void mb_copy(const uint4* src, uint4* dst, uin
 namespace launch {
 
 // which: 0 random CAS, 1 random add, 2 same-address add (one lane per wave),
-// 3 random coherent load, 4 random cached load, 5 copy (n words -> n words).
-// Returns the rate in operations/s (bytes/s for the copy, read + write).
+// 3 random coherent load, 4 random cached load, 5 copy (n words -> n words),
+// 6 coherent load pairs in one 128-byte line, 7 coherent load pairs in two
+// lines, 8 / 9 load word + load cell + CAS word with the cell in the word's
+// line / in another line.  Returns the rate in operations/s (bytes/s for the
+// copy, read + write; pairs for 6 and 7, reads for 8 and 9).
 hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint64_t iters, double* rate,
                       double* ms_out) {
   uint64_t* buf = nullptr;
@@ -85,6 +120,10 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
       case 2: hipLaunchKernelGGL(mb_add_same, dim3(blocks), dim3(256), 0, 0, buf, iters, sink); break;
       case 3: hipLaunchKernelGGL(mb_load_random, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 1, sink); break;
       case 4: hipLaunchKernelGGL(mb_load_random, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 0, sink); break;
+      case 6: hipLaunchKernelGGL(mb_load_pairs, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 1, sink); break;
+      case 7: hipLaunchKernelGGL(mb_load_pairs, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 0, sink); break;
+      case 8: hipLaunchKernelGGL(mb_ts_read, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 1, sink); break;
+      case 9: hipLaunchKernelGGL(mb_ts_read, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 0, sink); break;
       default: hipLaunchKernelGGL(mb_copy, dim3(2048), dim3(256), 0, 0, (const uint4*)buf, (uint4*)buf2, n_words / 2); break;
     }
   };

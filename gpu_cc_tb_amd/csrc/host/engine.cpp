@@ -1287,7 +1287,7 @@ int gcctb_sort_pairs(void* keys, uint32_t* vals, uint64_t n, int32_t key_bytes, 
 }
 
 int gcctb_microbench(int32_t which, uint64_t n_words, uint64_t threads, uint64_t iters, double* rate, double* ms) {
-  if (which < 0 || which > 5 || n_words < 2 || threads == 0 || iters == 0)
+  if (which < 0 || which > 9 || n_words < 16 || threads == 0 || iters == 0)
     return set_error(GCCTB_EINVAL, "bad microbench arguments");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(GCCTB_EDEVICE, "no HIP device");

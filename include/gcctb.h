@@ -232,6 +232,11 @@ int gcctb_sort_pairs(void* keys, uint32_t* vals, uint64_t n, int32_t key_bytes, 
  *         3: random coherent (agent-scope) 8-byte load
  *         4: random cached 8-byte load
  *         5: streaming copy of n_words words (rate in bytes/s, read + write)
+ *         6: pairs of random coherent 8-byte loads within one 128-byte line
+ *         7: pairs of random coherent 8-byte loads on two lines
+ *         8: load a word, load a cell in its 128-byte line, CAS the word
+ *         9: the same with the cell in another line
+ *            (rate in pairs/s for 6 and 7, reads/s for 8 and 9; n_words >= 16)
  * threads x iters operations; *rate in operations/s, *ms the timed launch. */
 int gcctb_microbench(int32_t which, uint64_t n_words, uint64_t threads, uint64_t iters, double* rate, double* ms);
 
