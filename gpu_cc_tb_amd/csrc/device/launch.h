@@ -24,9 +24,11 @@ hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t se
 // (every u32 IDX_EMPTY).  Keys must be unique and not IDX_EMPTY.
 hipError_t index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, uint32_t* buckets,
                         uint32_t nbuckets, hipStream_t s);
-// rows_out[i] = row of ops[i]'s key; also into rows_copy when it is given
+// rows_out[i] = row of ops[i]'s key; also into rows_copy when it is given;
+// ids_out[i] = what op_ids(ops, n, id_base) writes, when it is given
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                 hipStream_t s, uint32_t* rows_copy = nullptr);
+                 hipStream_t s, uint32_t* rows_copy = nullptr, uint32_t* ids_out = nullptr,
+                 uint32_t id_base = 0);
 
 // --- per-op helpers ---
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s);

@@ -134,13 +134,14 @@ __device__ __forceinline__ bool quad_bucket_match(const uint4& w, uint32_t key, 
 // once.  A probe whose bucket is full without the key goes on bucket by
 // bucket, the quad together.
 __global__ __launch_bounds__(256) void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                                               uint32_t* rows_copy) {
+                                               uint32_t* rows_copy, uint32_t* ids_out, uint32_t id_base) {
   const uint32_t q = threadIdx.x & 3u;
   const uint4* bkt = reinterpret_cast<const uint4*>(ix.buckets);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of 4
   for (uint64_t p0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~3ull; p0 < n; p0 += stride) {
     const uint64_t mine = p0 + q;
-    const uint32_t mykey = mine < n ? op_key(ops[mine]) : IDX_EMPTY;
+    const uint64_t myop = mine < n ? ops[mine] : 0ull;
+    const uint32_t mykey = mine < n ? op_key(myop) : IDX_EMPTY;
     const uint32_t key[4] = {quad_get<0>(mykey), quad_get<1>(mykey), quad_get<2>(mykey), quad_get<3>(mykey)};
     uint32_t b[4];
     uint4 w[4];
@@ -173,6 +174,7 @@ __global__ __launch_bounds__(256) void k_probe(IndexView ix, const uint64_t* ops
     if (mine < n) {
       rows_out[mine] = r;
       if (rows_copy) rows_copy[mine] = r;
+      if (ids_out) ids_out[mine] = (id_base + (uint32_t)mine) | (op_is_write(myop) ? OPID_WRITE : 0u);
     }
   }
 }
@@ -372,12 +374,17 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* total) {
   return Op::f(carry, x);
 }
 
+// Tile totals.  The operators are commutative, so the tile is read striped
+// across the block (element i * SC_BT + t by thread t: every load coalesced).
 template <class Op>
 __global__ __launch_bounds__(SC_BT) void k_scan_reduce(const uint32_t* in, uint64_t n, uint32_t* sums) {
-  const uint64_t base = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
+  const uint64_t base = (uint64_t)blockIdx.x * SC_TILE + threadIdx.x;
   uint32_t acc = Op::id();
-  for (int i = 0; i < SC_ITEMS; ++i)
-    if (base + i < n) acc = Op::f(acc, in[base + i]);
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    const uint64_t e = base + (uint64_t)i * SC_BT;
+    acc = Op::f(acc, e < n ? in[e] : Op::id());
+  }
   uint32_t total;
   block_scan<Op>(acc, &total);
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -404,35 +411,53 @@ __global__ __launch_bounds__(SC_BT) void k_scan_top(uint32_t* sums, uint32_t nb)
     }
   }
 }
+// Tile scans.  The tile moves between global memory and LDS striped across
+// the block (coalesced); thread t scans its SC_ITEMS consecutive elements out
+// of LDS.  Element j sits at LDS word j + j / 16, so the 16 consecutive words
+// of thread t start at 17 t and a wave's reads fall in distinct banks.  The
+// whole tile is loaded before any of it is stored, so in may equal out.
 template <class Op, bool INCLUSIVE>
 __global__ __launch_bounds__(SC_BT) void k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n,
                                                      const uint32_t* sums) {
-  const uint64_t base = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
+  static_assert(SC_ITEMS == 16, "the LDS padding assumes 16 items per thread");
+  __shared__ uint32_t tile[SC_TILE + SC_TILE / 16];
+  __shared__ uint32_t tincl[SC_BT];
+  const uint32_t t = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    const uint32_t j = (uint32_t)i * SC_BT + t;
+    tile[j + (j >> 4)] = base + j < n ? in[base + j] : Op::id();
+  }
+  __syncthreads();
   uint32_t vals[SC_ITEMS];
   uint32_t acc = Op::id();
 #pragma unroll
   for (int i = 0; i < SC_ITEMS; ++i) {
-    vals[i] = (base + i < n) ? in[base + i] : Op::id();
+    vals[i] = tile[t * 17 + i];
     acc = Op::f(acc, vals[i]);
   }
   uint32_t total;
   const uint32_t incl = block_scan<Op>(acc, &total);
   // exclusive prefix for this thread = block carry + (incl of previous threads)
-  __shared__ uint32_t tincl[SC_BT];
-  tincl[threadIdx.x] = incl;
+  tincl[t] = incl;
   __syncthreads();
-  uint32_t c = Op::f(sums[blockIdx.x], threadIdx.x == 0 ? Op::id() : tincl[threadIdx.x - 1]);
+  uint32_t c = Op::f(sums[blockIdx.x], t == 0 ? Op::id() : tincl[t - 1]);
 #pragma unroll
   for (int i = 0; i < SC_ITEMS; ++i) {
-    if (base + i < n) {
-      if (INCLUSIVE) {
-        c = Op::f(c, vals[i]);
-        out[base + i] = c;
-      } else {
-        out[base + i] = c;
-        c = Op::f(c, vals[i]);
-      }
+    if (INCLUSIVE) {
+      c = Op::f(c, vals[i]);
+      tile[t * 17 + i] = c;
+    } else {
+      tile[t * 17 + i] = c;
+      c = Op::f(c, vals[i]);
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SC_ITEMS; ++i) {
+    const uint32_t j = (uint32_t)i * SC_BT + t;
+    if (base + j < n) out[base + j] = tile[j + (j >> 4)];
   }
 }
 
@@ -641,8 +666,8 @@ hipError_t index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, 
   GC_LAUNCH(k_index_insert, n, keys, vals, n, buckets, nbuckets);
 }
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                 hipStream_t s, uint32_t* rows_copy) {
-  GC_LAUNCH(k_probe, n, ix, ops, n, rows_out, rows_copy);
+                 hipStream_t s, uint32_t* rows_copy, uint32_t* ids_out, uint32_t id_base) {
+  GC_LAUNCH(k_probe, n, ix, ops, n, rows_out, rows_copy, ids_out, id_base);
 }
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s) {
   GC_LAUNCH(k_op_tx, ntx, tx_off, ntx, op_tx);

@@ -292,8 +292,7 @@ class GaccoHost : public SchemeHost {
     const uint64_t n = o1 - o0;
     if (n == 0) return;
     hipStream_t s = ctx.stream;
-    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.rows_out.p + o0, s, ctx.sk0.p));
-    HIP_TRY(launch::op_ids(ctx.ops.p, n, (uint32_t)o0, ctx.sv0.p, s));
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.rows_out.p + o0, s, ctx.sk0.p, ctx.sv0.p, (uint32_t)o0));
     uint32_t *srow, *sop;
     // rows are below cfg.rows; a miss (IDX_MISS) has all row bits set and sorts last
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.cfg.rows),
@@ -320,8 +319,7 @@ class GPUTxHost : public SchemeHost {
     const uint64_t n = ctx.nops, ntx = ctx.ntx;
     hipStream_t s = ctx.stream;
     // per-row access lists in (row, tx) order
-    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p));
-    HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p, ctx.sv0.p, 0));
     uint32_t *srow, *sop;
     // rows are below cfg.rows; a miss (IDX_MISS) has all row bits set and sorts last
     const int row_bits = bits_for(ctx.cfg.rows);
