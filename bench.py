@@ -59,7 +59,8 @@ def parse():
     p.add_argument("--bs", type=int, default=-1, help="block_size; -1 = autotuned per scheme")
     p.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     p.add_argument("--serve-rows", type=int, default=67108864)
-    p.add_argument("--serve-txs", type=int, default=262144, help="requests per serving run")
+    p.add_argument("--serve-txs", type=int, default=1000000,
+                   help="requests per serving run (enough launches for stable percentiles)")
     p.add_argument("--serve-load", type=float, default=0.5, help="offered load, fraction of measured peak")
     p.add_argument("--serve-schemes", default="mvcc,silo")
     p.add_argument("--no-serve", action="store_true")
