@@ -275,20 +275,32 @@ struct TwoPL {
       }
       commit = (c.ph == PH_COMMIT);
     }
-    uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, commit && (lane_id() & (G - 1)) == 0);
+    // Every lock is held: the cell accesses and the commit ticket both fall
+    // inside that window in either order (a conflicting transaction locks
+    // only after our unlocks, so its ticket is later), so they are issued
+    // together and waited for once.
+    uint64_t v[K];
     if (commit) {
-      ticket = group_bcast64<G>(ticket);
       order_acquire();
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const uint32_t m = c.meta[k];
+        v[k] = 0;
         if (!(m & M_VALID)) continue;
         uint64_t* cell = cell_ptr(a, c.row[k], m);
-        const uint32_t j = op_index<G>(k);
         if (m & M_WRITE)
-          st(cell, write_value(a.seed, c.tx, j));
+          st(cell, write_value(a.seed, c.tx, op_index<G>(k)));
         else
-          a.reads[c.op0 + j] = ld(cell);
+          v[k] = ld(cell);
+      }
+    }
+    uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, commit && (lane_id() & (G - 1)) == 0);
+    if (commit) {
+      ticket = group_bcast64<G>(ticket);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint32_t m = c.meta[k];
+        if ((m & M_VALID) && !(m & M_WRITE)) a.reads[c.op0 + op_index<G>(k)] = v[k];
       }
       order_release();
 #pragma unroll
