@@ -1361,8 +1361,12 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups)
   }
 }
 
-template <class S, int G, bool PERSIST>
-__global__ __launch_bounds__(1024) void txn_exec(ExecArgs a) {
+// MAXT: the largest block the instantiation is launched with.  A block of
+// 1,024 threads holds 4 waves per SIMD, which caps a lane at 128 registers;
+// the persistent form's tuned shapes use blocks of at most 256 threads, and
+// their instantiation may use up to 512 registers instead of spilling.
+template <class S, int G, bool PERSIST, int MAXT = 1024>
+__global__ __launch_bounds__(MAXT) void txn_exec(ExecArgs a) {
   TxCtx<G> c;
   if constexpr (PERSIST) {
     run_persistent<S, G>(c, a, ((uint64_t)gridDim.x * blockDim.x) / G);

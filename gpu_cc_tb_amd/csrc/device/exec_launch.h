@@ -8,8 +8,9 @@
 namespace gcctb {
 namespace dev {
 
-template <class S, int G, bool P>
+template <class S, int G, bool P, int MAXT>
 hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s) {
+  if (threads > MAXT) return hipErrorInvalidValue;
   const uint64_t total = nslots * (uint64_t)G;
   uint64_t blocks = (total + threads - 1) / threads;
   if (P) {
@@ -19,7 +20,7 @@ hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&txn_exec<S, G, P>),
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&txn_exec<S, G, P, MAXT>),
                                                        threads, 0);
     if (e != hipSuccess) return e;
     const uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 1) * (uint64_t)(cus > 0 ? cus : 1);
@@ -32,7 +33,7 @@ hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream
     }
   }
   if (blocks == 0 || blocks > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((txn_exec<S, G, P>), dim3((unsigned)blocks), dim3((unsigned)threads), 0, s, a);
+  hipLaunchKernelGGL((txn_exec<S, G, P, MAXT>), dim3((unsigned)blocks), dim3((unsigned)threads), 0, s, a);
   return hipGetLastError();
 }
 
@@ -42,10 +43,13 @@ hipError_t launch_exec_g(int G, const ExecArgs& args, uint64_t nslots, int threa
   if (threads <= 0 || threads > 1024) return hipErrorInvalidValue;
   ExecArgs a = args;
   a.persist = persist ? 1u : 0u;
+  // persistent launches of at most 256 threads per block take the
+  // instantiation bounded by that block size (more registers, no spills)
 #define GC_CASE(g)                                                                  \
   case g:                                                                           \
-    return persist ? launch_one<S, g, true>(a, nslots, threads, s)                  \
-                   : launch_one<S, g, false>(a, nslots, threads, s);
+    return persist ? (threads <= 256 ? launch_one<S, g, true, 256>(a, nslots, threads, s)    \
+                                     : launch_one<S, g, true, 1024>(a, nslots, threads, s))  \
+                   : launch_one<S, g, false, 1024>(a, nslots, threads, s);
   switch (G) {
     GC_CASE(1)
     GC_CASE(2)
