@@ -84,6 +84,10 @@ struct gcctb_ctx {
   int threads = 1024;
   uint64_t watchdog_ticks = 0;
   uint32_t urgent_attempts = 256;
+  // persistent and serving forms: rows from a probe pass before the launch
+  // (1), looked up by each group as it takes a transaction (0), or the
+  // scheme's measured default (-1; GCCTB_PROBE_PASS sets 0 or 1)
+  int probe_pass = -1;
   uint32_t backoff_exp = 10;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
@@ -169,6 +173,11 @@ class SchemeHost {
   // True when PrepStream / PrepLaunch leave the row of every op they cover in
   // rows_out, so the persistent and serving forms need no probe pass of their own.
   virtual bool ProbesRows() const { return false; }
+  // Persistent and serving forms: whether a probe pass before the launch
+  // beats looking rows up in the hash index as each transaction is taken
+  // (one bucket read on the refill chain).  Measured on the low-contention
+  // workload: only MVCC gains from the pass.
+  virtual bool WantsProbePass() const { return false; }
   virtual void Init(int batch_id, uint64_t batch_st) {
     (void)batch_id;
     (void)batch_st;
@@ -253,6 +262,7 @@ class MVCCHost : public SchemeHost {
     HIP_TRY(launch::exec_mvcc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
   uint64_t GetMemSize() const override { return ctx.cc.n * 8 + ctx.vprev.n * 4 + ctx.pool.n * 8; }
+  bool WantsProbePass() const override { return true; }
 };
 class SiloHost : public SchemeHost {
  public:
@@ -491,6 +501,7 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
     // persistent form: aborts after which a transaction pauses admission
     // until it commits (GCCTB_URGENT_ATTEMPTS, 0 = never)
     if (const char* ua = std::getenv("GCCTB_URGENT_ATTEMPTS")) c->urgent_attempts = (uint32_t)std::atoi(ua);
+    if (const char* pp = std::getenv("GCCTB_PROBE_PASS")) c->probe_pass = std::atoi(pp) != 0 ? 1 : 0;
     // retry delay: at most 2^GCCTB_BACKOFF_EXP units (0..16, default 10)
     if (const char* be = std::getenv("GCCTB_BACKOFF_EXP")) {
       const int e = std::atoi(be);
@@ -775,14 +786,16 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (form == FORM_PERSIST) {
       c->host->PrepStream();
-      // Every op's row in one probe pass (GaccO's and GPUTx's preparation
-      // already made it): a group loading its next transaction then reads
-      // rows instead of walking the index while the other groups of its wave wait.
-      if (!c->host->ProbesRows() && c->nops)
+      // Rows: GaccO's and GPUTx's preparation already has them; for the
+      // other schemes a probe pass finds every op's row first, or each group
+      // looks its rows up as it takes a transaction (WantsProbePass).
+      const bool pass = c->probe_pass >= 0 ? c->probe_pass != 0 : c->host->WantsProbePass();
+      const bool probed = c->host->ProbesRows() || pass;
+      if (!c->host->ProbesRows() && pass && c->nops)
         HIP_TRY(launch::probe(c->ix, c->ops.p, c->nops, c->rows_out.p, s));
       HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
-      c->rows_probed = true;
+      c->rows_probed = probed;
       c->host->Launch(0, 0, c->ntx, true);
       c->rows_probed = false;
       HIP_TRY(hipEventRecord(c->ev[4], s));
@@ -1038,10 +1051,11 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
       const auto tl = clk::now();
       if (o1 > o0) HIP_TRY(hipMemcpyAsync(c->ops.p + o0, host_ops.p + o0, (o1 - o0) * 8, hipMemcpyHostToDevice, s));
       c->host->PrepLaunch(o0, o1);
-      if (o1 > o0 && !c->host->ProbesRows())
+      const bool pass = c->probe_pass >= 0 ? c->probe_pass != 0 : c->host->WantsProbePass();
+      if (o1 > o0 && !c->host->ProbesRows() && pass)
         HIP_TRY(launch::probe(c->ix, c->ops.p + o0, o1 - o0, c->rows_out.p + o0, s));
       HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
-      c->rows_probed = true;
+      c->rows_probed = c->host->ProbesRows() || pass;
       c->host->Launch((int)launches, lo, hi, true);
       c->rows_probed = false;
       c->host->Post((int)launches, lo, hi - lo);
