@@ -1128,21 +1128,67 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
     c.op0 = a.tx_off[c.tx];
     c.nops = (uint32_t)(a.tx_off[c.tx + 1] - c.op0);
   }
+  // Every slot's op, then every slot's row, each step issued for all slots
+  // before its wait: rows from the probe pass, or from the hash index (all
+  // home buckets read at once; a key whose home bucket is full without it
+  // walks on through index_find).
+  uint64_t opv[K];
+  uint32_t rowv[K];
+  bool has[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const uint32_t j = op_index<G>(k);
+    has[k] = c.live && op_index<G>(k) < c.nops;
+    opv[k] = has[k] ? a.ops[c.op0 + op_index<G>(k)] : 0ull;
+  }
+  if (a.rows_in) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) rowv[k] = has[k] ? a.rows_in[c.op0 + op_index<G>(k)] : IDX_MISS;
+  } else {
+    uint4 q0[K], q1[K];
+    const uint32_t* bk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t key = op_key(opv[k]);
+      bk[k] = a.index.buckets + (size_t)idx_bucket(key, a.index.nbuckets) * (2 * IDX_SLOTS);
+      if (has[k] && key != IDX_EMPTY) {
+        q0[k] = reinterpret_cast<const uint4*>(bk[k])[0];
+        q1[k] = reinterpret_cast<const uint4*>(bk[k])[1];
+      } else {
+        q0[k] = q1[k] = make_uint4(IDX_EMPTY, IDX_EMPTY, IDX_EMPTY, IDX_EMPTY);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      rowv[k] = IDX_MISS;
+      if (!has[k]) continue;
+      const uint32_t key = op_key(opv[k]);
+      if (key == IDX_EMPTY) continue;
+      const uint32_t ks[IDX_SLOTS] = {q0[k].x, q0[k].y, q0[k].z, q0[k].w, q1[k].x, q1[k].y, q1[k].z, q1[k].w};
+      int hit = -1;
+      bool free_slot = false;
+#pragma unroll
+      for (int i = 0; i < IDX_SLOTS; ++i) {
+        hit = ks[i] == key ? i : hit;
+        free_slot |= ks[i] == IDX_EMPTY;
+      }
+      if (hit >= 0)
+        rowv[k] = bk[k][IDX_SLOTS + hit];
+      else if (!free_slot)
+        rowv[k] = index_find(a.index, key);  // the home bucket is full: walk on
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (has[k]) a.rows_out[c.op0 + op_index<G>(k)] = rowv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
     c.meta[k] = 0;
     c.row[k] = 0;
     c.st[k] = 0;
-    if (c.live && j < c.nops) {
-      const uint64_t op = a.ops[c.op0 + j];
-      uint32_t row;
-      if (a.rows_in) {
-        row = a.rows_in[c.op0 + j];
-      } else {
-        row = index_find(a.index, op_key(op));
-        a.rows_out[c.op0 + j] = row;
-      }
+    if (has[k]) {
+      const uint32_t j = op_index<G>(k);
+      const uint64_t op = opv[k];
+      const uint32_t row = rowv[k];
       if (row < a.rows) {
         c.row[k] = row;
         c.meta[k] = M_VALID | (op_is_write(op) ? M_WRITE : 0u) | (op_col(op) << M_COL_SHIFT);
