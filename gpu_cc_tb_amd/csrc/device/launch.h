@@ -53,7 +53,6 @@ hipError_t radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1
 
 // --- scans (u32) --- tmp must hold scan_tmp_words(n) u32.
 uint64_t scan_tmp_words(uint64_t n);
-hipError_t exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
 hipError_t inclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
 hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
 

@@ -696,9 +696,6 @@ hipError_t radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1
                           hipStream_t s) {
   return radix_impl<uint64_t>(k0, v0, k1, v1, n, key_bits, tmp, keys_out, vals_out, s);
 }
-hipError_t exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
-  return scan_impl<OpAdd, false>(in, out, n, tmp, s);
-}
 hipError_t inclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
   return scan_impl<OpAdd, true>(in, out, n, tmp, s);
 }
