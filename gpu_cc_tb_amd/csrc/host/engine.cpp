@@ -130,6 +130,7 @@ struct gcctb_ctx {
   gcctb::DevBuf<uint64_t> tcommit;  // serving form: per-tx commit tick
   bool record_commit = false;
   std::vector<uint32_t> h_rank_counts;
+  bool rank_counts_ready = false;  // h_rank_counts matches rank[]
   std::vector<uint64_t> custom_batches;  // uneven batch sizes (empty: batch_size)
   // hipGraph of the per-batch sequence (graph form), rebuilt when inputs change
   hipGraphExec_t graph_exec = nullptr;
@@ -324,8 +325,35 @@ class GaccoHost : public SchemeHost {
 class GPUTxHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
+  // plain and graph forms: ranks, then one batch per rank level
   void Init0() override {
+    rank_all();
+    rank_counts(ctx);
     batches.clear();
+    for (uint32_t c : ctx.h_rank_counts) batches.push_back(c);
+  }
+  // per-rank counts (gputx_pre3, src/gputx.cu:30-38), copied to the host
+  static void rank_counts(gcctb_ctx& c) {
+    if (c.rank_counts_ready) return;
+    hipStream_t s = c.stream;
+    c.h_rank_counts.assign(c.nranks, 0);
+    if (c.nops && c.nranks) {
+      DevBuf<uint32_t> hist;
+      hist.alloc(c.nranks);
+      HIP_TRY(launch::fill_u32(hist.p, c.nranks, 0, s));
+      HIP_TRY(launch::rank_hist(c.rank.p, c.ntx, (uint32_t)c.nranks, hist.p, s));
+      HIP_TRY(hipMemcpyAsync(c.h_rank_counts.data(), hist.p, c.nranks * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+    } else if (c.nranks) {
+      // no ops at all: every transaction has rank 0
+      c.h_rank_counts[0] = (uint32_t)c.ntx;
+    }
+    c.rank_counts_ready = true;
+  }
+  // rank[] of every transaction, tx_map in (rank, tx) order, and need[] /
+  // counters[] for execution in (row, rank, tx) order
+  void rank_all() {
+    ctx.rank_counts_ready = false;
     const uint64_t n = ctx.nops, ntx = ctx.ntx;
     hipStream_t s = ctx.stream;
     // per-row access lists in (row, tx) order
@@ -345,22 +373,8 @@ class GPUTxHost : public SchemeHost {
     uint32_t max_rank = 0;
     HIP_TRY(hipMemcpyAsync(&max_rank, ctx.max_rank.p, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    ctx.nranks = n ? (uint64_t)max_rank + 1 : 0;
-    // per-rank counts (gputx_pre3, src/gputx.cu:30-38) -> batches
-    ctx.h_rank_counts.assign(ctx.nranks, 0);
-    if (ctx.nranks) {
-      DevBuf<uint32_t> hist;
-      hist.alloc(ctx.nranks);
-      HIP_TRY(launch::fill_u32(hist.p, ctx.nranks, 0, s));
-      HIP_TRY(launch::rank_hist(ctx.rank.p, ntx, (uint32_t)ctx.nranks, hist.p, s));
-      HIP_TRY(hipMemcpyAsync(ctx.h_rank_counts.data(), hist.p, ctx.nranks * 4, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-    } else if (ntx) {
-      // no ops at all: every transaction has rank 0
-      ctx.nranks = 1;
-      ctx.h_rank_counts.assign(1, (uint32_t)ntx);
-    }
-    for (uint32_t c : ctx.h_rank_counts) batches.push_back(c);
+    // no ops at all: every transaction has rank 0
+    ctx.nranks = n ? (uint64_t)max_rank + 1 : (ntx ? 1 : 0);
     // transactions in (rank, tx) order: stable sort of ranks (include/gputx.cuh:281)
     const int rank_bits = std::max(1, bits_for(ctx.nranks ? ctx.nranks - 1 : 0));
     HIP_TRY(hipMemcpyAsync(ctx.sk0.p, ctx.rank.p, ntx * 4, hipMemcpyDeviceToDevice, s));
@@ -405,8 +419,9 @@ class GPUTxHost : public SchemeHost {
     HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows, s));
   }
   void Init(int, uint64_t) override {}
-  // the persistent form runs every rank level in one launch
-  void PrepStream() override { Init0(); }
+  // the persistent form runs every rank level in one launch: it needs no
+  // per-rank counts (and no host round trip for them)
+  void PrepStream() override { rank_all(); }
   bool ProbesRows() const override { return true; }
   void PostStream() override {}
   void Launch(int, uint64_t lo, uint64_t hi, bool persist) override {
@@ -1165,7 +1180,10 @@ int gcctb_fetch_ranks(gcctb_ctx* c, uint32_t* rank, uint32_t* counts, uint64_t* 
   if (c->scheme != SCHEME_GPUTX) return set_error(GCCTB_ESTATE, "ranks exist only for gputx");
   try {
     if (rank && c->ntx) HIP_TRY(hipMemcpy(rank, c->rank.p, c->ntx * 4, hipMemcpyDeviceToHost));
-    if (counts) std::memcpy(counts, c->h_rank_counts.data(), c->h_rank_counts.size() * 4);
+    if (counts) {
+      gcctb::GPUTxHost::rank_counts(*c);
+      std::memcpy(counts, c->h_rank_counts.data(), c->h_rank_counts.size() * 4);
+    }
     if (nranks) *nranks = c->nranks;
   } catch (const HipFailure& f) {
     return hip_fail(f);

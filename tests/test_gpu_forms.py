@@ -172,3 +172,27 @@ def test_gputx_wide_order_keys(form):
         res = ctx.fetch()
     assert st["committed"] == ntx and st["ranks"] == nranks
     check_replay(w, res, rows, g.SEED)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["hot", "varlen"])
+def test_gputx_ranks_after_persistent(case):
+    """The persistent form ranks without copying per-rank counts to the
+    host; fetching them afterwards still gives the oracle's ranks and
+    counts (src/gputx.cu:5-38), also after a plain run of other inputs."""
+    rows, ntx, lmin, lmax, theta, rr, batch, wd, bs = FORM_CASES[case]
+    print(f"[gputx-ranks] case={case}", flush=True)
+    w = g.generate(rows, ntx, theta=theta, read_ratio=rr, len_min=lmin, len_max=lmax)
+    w0 = g.generate(rows, ntx // 2, theta=0.0, read_ratio=0.5, len_min=4, len_max=4, seed=7)
+    rank_o, counts_o = orc.gputx_rank(rows, w.ops, w.tx_off)
+    with g.Context("gputx", rows=rows, warp_density=wd, block_size=bs, batch_size=batch) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w0)
+        ctx.run_plain()
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run("persistent")
+        rank_g, counts_g = ctx.fetch_ranks()
+    assert st["committed"] == w.ntx and st["ranks"] == counts_o.size
+    assert np.array_equal(rank_g, rank_o)
+    assert np.array_equal(counts_g, counts_o)
