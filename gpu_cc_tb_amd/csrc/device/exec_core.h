@@ -1087,12 +1087,12 @@ struct Ordered {
     } else {
       c.m_wait += now - c.t_round;
       // An op `gap` turns short of its own cannot run before `gap` handoffs
-      // on its row: rest until then (POLL_TICKS_PER_TURN each, at most
+      // on its row: rest until then (a.poll_ticks each, at most
       // POLL_REST_MAX) instead of polling every round; deep waiters on a hot
       // row then stop flooding the memory system the row's chain depends on.
       const uint64_t turns = group_min64<G>(gap);
       if (turns > 1) {
-        const uint64_t rest = (turns - 1) * POLL_TICKS_PER_TURN;
+        const uint64_t rest = (turns - 1) * (uint64_t)a.poll_ticks;
         c.t_resume = now + (rest < POLL_REST_MAX ? rest : POLL_REST_MAX);
       }
     }
@@ -1101,7 +1101,6 @@ struct Ordered {
   // Refill at once: a done group that waited for its wave would idle behind
   // groups queued on hot rows (measured: 29% slower on the batched stream).
   static constexpr bool kBulkRefill = false;
-  static constexpr uint64_t POLL_TICKS_PER_TURN = 50;  // 0.5 us per handoff on the row
   static constexpr uint64_t POLL_REST_MAX = 12800;     // 128 us
 };
 

@@ -264,6 +264,7 @@ struct ExecArgs {
   uint32_t persist;        // 1 in persistent launches (slots from CTR_SLOT, gates, urgent state)
   uint32_t urgent_attempts; // persistent form: aborts that make a transaction urgent (0 = never)
   uint32_t backoff_exp;     // retry delay spans at most 2^backoff_exp units
+  uint32_t poll_ticks;      // ordered schemes: expected ticks per handoff on a row (rest pacing)
   uint64_t* tcommit;       // optional, per tx: wall-clock tick of its commit (serving form)
 };
 

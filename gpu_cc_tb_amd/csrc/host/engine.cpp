@@ -89,6 +89,7 @@ struct gcctb_ctx {
   // scheme's measured default (-1; GCCTB_PROBE_PASS sets 0 or 1)
   int probe_pass = -1;
   uint32_t backoff_exp = 10;
+  uint32_t poll_ticks = 25;  // 0.25 us (measured sweep 6..100 at theta 0.99)
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   // table + index
@@ -231,6 +232,7 @@ class SchemeHost {
     a.watchdog_ticks = ctx.watchdog_ticks;
     a.urgent_attempts = ctx.urgent_attempts;
     a.backoff_exp = ctx.backoff_exp;
+    a.poll_ticks = ctx.poll_ticks;
     a.dbg = ctx.dbg.p;
     a.persist_groups = ctx.cfg.persist_window;
     a.adapt = ctx.cfg.persist_adaptive ? 1u : 0u;
@@ -521,6 +523,12 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
     if (const char* be = std::getenv("GCCTB_BACKOFF_EXP")) {
       const int e = std::atoi(be);
       c->backoff_exp = (uint32_t)(e < 0 ? 0 : e > 16 ? 16 : e);
+    }
+    // GaccO / GPUTx: assumed handoff time per turn on a row, in 10 ns ticks,
+    // that paces a waiter's rest (GCCTB_POLL_TICKS, 1..1000, default 25)
+    if (const char* pt = std::getenv("GCCTB_POLL_TICKS")) {
+      const int t = std::atoi(pt);
+      c->poll_ticks = (uint32_t)(t < 1 ? 1 : t > 1000 ? 1000 : t);
     }
     const char* dump = std::getenv("GCCTB_DEBUG_DUMP");
     if (dump && *dump) c->dbg_path = dump;
