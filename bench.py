@@ -89,6 +89,36 @@ def traffic_per_launch():
     return None
 
 
+def atomic_roofline(args, avg_launch_ms):
+    """Atomic throughput of the execution kernel against the random 64-bit
+    CAS ceiling, measured here on this GPU over a rows x 8 B array."""
+    ceiling = g.microbench("cas_random", args.rows, threads=1 << 20, iters=16)["rate"]
+    per_launch = atomics_per_launch()
+    achieved = per_launch / (avg_launch_ms / 1e3) if per_launch else None
+    return {
+        "bound": "atomic",
+        "achieved": round(achieved) if achieved else None,
+        "ceiling": round(ceiling),
+        "unit": "atomics/s",
+        "frac": round(achieved / ceiling, 5) if achieved else None,
+        "per_launch": per_launch,
+        "ceiling_kernel": f"random 64-bit CAS over {args.rows} x 8 B (gcctb_microbench cas_random)",
+    }
+
+
+def atomics_per_launch():
+    """Atomic requests per launch of the execution kernel on this workload,
+    from the committed rocprofv3 PMC summary (TCC_ATOMIC_sum, scaled by what
+    the counter reads for the random-CAS kernel's known count), or None."""
+    path = os.path.join(PROFILES, "txn_exec_config2_atomics.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("atomics_per_launch:"):
+            return float(line.split(":", 1)[1].strip().split()[0])
+    return None
+
+
 def cpu_baseline(w_full: g.Ops, args, schemes):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
@@ -250,6 +280,7 @@ def main():
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
+        "atomic_roofline": atomic_roofline(args, avg_launch_ms),
         "device": info["name"],
         "gen_s": round(gen_s, 2),
     }
