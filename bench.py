@@ -201,7 +201,7 @@ def main():
     gen_s = time.time() - t0
     info = g.device_info()
     per = {}
-    b_ms = b_kern = p_ms = 0.0
+    b_ms = b_kern = p_ms = p_kern = 0.0
     b_launch = 0
     for s in schemes:
         shape = autotune.lookup(s, args.theta, args.read_ratio, table)
@@ -210,7 +210,7 @@ def main():
         # the plain form runs at its own best low-contention shape
         pwd, pbs = (wd, bs) if args.wd >= 0 or args.bs >= 0 else autotune.DEFAULT_SHAPE[s]
         plain_here = not args.no_plain and (pwd, pbs) == (wd, bs)
-        sp_ms = 0.0
+        sp_ms = sp_kern = 0.0
         with g.Context(s, rows=args.rows, warp_density=wd, block_size=bs, batch_size=args.batch,
                        persist_window=shape["window"], persist_adaptive=shape["adaptive"]) as ctx:
             ctx.seed_table()
@@ -221,19 +221,20 @@ def main():
                     ctx.run("plain")
             sb_ms, sb_kern, sb_launch, sb_abort = run_form(ctx, "persistent", args.steps)
             if plain_here:
-                sp_ms, _, _, _ = run_form(ctx, "plain", args.steps)
+                sp_ms, sp_kern, _, _ = run_form(ctx, "plain", args.steps)
         if not args.no_plain and not plain_here:
             with g.Context(s, rows=args.rows, warp_density=pwd, block_size=pbs, batch_size=args.batch) as ctx:
                 ctx.seed_table()
                 ctx.load_ops(w)
                 for _ in range(args.warmup):
                     ctx.run("plain")
-                sp_ms, _, _, _ = run_form(ctx, "plain", args.steps)
+                sp_ms, sp_kern, _, _ = run_form(ctx, "plain", args.steps)
         per[s] = {
             "batched_txps": round(args.txs * args.steps / (sb_ms / 1e3)),
             "plain_txps": round(args.txs * args.steps / (sp_ms / 1e3)) if sp_ms else None,
             "batched_ms_per_pass": round(sb_ms / args.steps, 3),
             "kernel_ms_per_pass": round(sb_kern / args.steps, 3),
+            "plain_kernel_ms_per_pass": round(sp_kern / args.steps, 3) if sp_ms else None,
             "aborts_per_pass": sb_abort // args.steps,
             "wd": wd, "bs": bs, "window": shape["window"], "plain_wd": pwd, "plain_bs": pbs,
         }
@@ -241,11 +242,13 @@ def main():
         b_kern += sb_kern
         b_launch += sb_launch
         p_ms += sp_ms
+        p_kern += sp_kern
     committed = args.txs * args.steps * len(schemes)
     value = committed / (b_ms / 1e3)
     bytes_launch = algorithmic_bytes(args.txs, w.nops, nreads)
     avg_launch_ms = b_kern / max(1, b_launch)
     achieved = bytes_launch / (avg_launch_ms / 1e3) / 1e9
+    p_achieved = bytes_launch * args.steps * len(schemes) / (p_kern / 1e3) / 1e9 if p_kern else 0.0
     line = {
         "metric": "committed Tx/s on 1 MI355X (batched vs plain) + p99 commit latency",
         "value": round(value),
@@ -267,7 +270,13 @@ def main():
             "read_ratio": args.read_ratio, "batch_size": args.batch, "schemes": schemes,
         },
         "plain": {"value": round(committed / (p_ms / 1e3)) if p_ms else None,
-                  "batched_over_plain": round(p_ms / b_ms, 3) if p_ms else None},
+                  "batched_over_plain": round(p_ms / b_ms, 3) if p_ms else None,
+                  # the same algorithmic bytes per pass, over the plain form's kernel time
+                  # (every scheme's launches of one pass together)
+                  "roofline": ({"bound": "hbm", "achieved": round(p_achieved, 2), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(p_achieved / HBM_PEAK_GBS, 5),
+                                "kernel_ms_per_pass": round(p_kern / args.steps / len(schemes), 4)}
+                               if p_kern else None)},
         "per_scheme": per,
         "roofline": {
             "bound": "hbm",
