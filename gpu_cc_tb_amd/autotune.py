@@ -37,7 +37,7 @@ WD = [2, 3, 4]
 BS = [4, 8]
 # admission windows (transactions in flight; 0 = device-filling).  The
 # adaptive window (-1) is left out: the library refuses it in this build.
-WINDOWS = [0, 32768, 16384, 8192, 4096, 1024, 256]
+WINDOWS = [0, 32768, 16384, 8192, 4096, 1024, 512, 256]
 
 # starting shapes (from sweeps of the plain form at low contention)
 DEFAULT_SHAPE = {"tpl_nw": (3, 8), "tpl_wd": (3, 8), "to": (4, 8), "mvcc": (4, 4), "silo": (3, 8),
