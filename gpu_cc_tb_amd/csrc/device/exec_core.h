@@ -1276,7 +1276,7 @@ __device__ __forceinline__ void adapt_window(const ExecArgs& a, uint64_t now, ui
     ab += ld(a.ctrs + CTR_STAT0 + 16 * s + 1);
   }
   const uint64_t dc = cm - ld(ctl + 1), da = ab - ld(ctl + 2);
-  if (dc + da < 256) return;  // too few events to judge
+  if (dc + da < 256) return;  // too few events to decide on
   st(ctl + 1, cm);
   st(ctl + 2, ab);
   uint64_t w = ld(a.ctrs + CTR_GATE0);
