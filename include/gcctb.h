@@ -170,7 +170,8 @@ int gcctb_set_batches(gcctb_ctx* ctx, const uint64_t* sizes, uint64_t n);
 int gcctb_fetch(gcctb_ctx* ctx, uint64_t* table, uint64_t* reads, uint64_t* order, uint32_t* aborts);
 /* gputx: per-transaction rank (src/gputx.cu:5-27) and per-rank counts
  * (include/gputx.cuh:283-294).  counts may be null; nranks receives the level
- * count. */
+ * count.  After a persistent run, which needs no per-rank counts, the first
+ * call that asks for them computes them on the device. */
 int gcctb_fetch_ranks(gcctb_ctx* ctx, uint32_t* rank, uint32_t* counts, uint64_t* nranks);
 /* Per-row CC words after a run (2PL/T-O/MVCC/Silo/TicToc: one u64 per row). */
 int gcctb_fetch_cc(gcctb_ctx* ctx, uint64_t* words);
