@@ -66,6 +66,10 @@ hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_
 hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
                        uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
                        uint64_t nrows, hipStream_t s);
+// need[op] = gstart[p] - segstart[p] for op = sop[p] & OPID_MASK: the turn
+// relative to the row's first access, for counters that start at zero.
+hipError_t access_need_rel(const uint32_t* sop, const uint32_t* gstart, const uint32_t* segstart, uint64_t n,
+                           uint32_t* need, hipStream_t s);
 // GPUTx: *conflicts += adjacent same-row, same-rank accesses where either
 // writes, from the (row << rank_bits | rank) sorted keys and the group flags.
 hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,

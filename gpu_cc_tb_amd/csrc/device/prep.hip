@@ -526,6 +526,13 @@ __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const u
   }
 }
 
+__global__ void k_access_need_rel(const uint32_t* sop, const uint32_t* gstart, const uint32_t* segstart,
+                                  uint64_t n, uint32_t* need) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x)
+    need[sop[p] & OPID_MASK] = gstart[p] - segstart[p];
+}
+
 // GPUTx same-rank conflicts: adjacent accesses of one row with equal rank
 // where either writes.  In the (row << rank_bits | rank) sorted keys, same
 // row and rank means equal keys; flags[p] (access_group_flags) is non-zero
@@ -710,6 +717,10 @@ hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t
                        uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
                        uint64_t nrows, hipStream_t s) {
   GC_LAUNCH(k_access_need, n, srow, sop, gstart, n, pos_base, need, counters, nrows);
+}
+hipError_t access_need_rel(const uint32_t* sop, const uint32_t* gstart, const uint32_t* segstart, uint64_t n,
+                           uint32_t* need, hipStream_t s) {
+  GC_LAUNCH(k_access_need_rel, n, sop, gstart, segstart, n, need);
 }
 hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
                          hipStream_t s) {

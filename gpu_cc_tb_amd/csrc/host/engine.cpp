@@ -411,14 +411,19 @@ class GPUTxHost : public SchemeHost {
       HIP_TRY(launch::gtx_conflicts(skey, ctx.flags.p, n, ctx.conflicts.p, s));
     }
   }
-  // need[] and counters[] from accesses in execution order (group flags in
-  // ctx.flags, which the conflict count reads afterwards)
+  // need[] from accesses in execution order (group flags in ctx.flags, which
+  // the conflict count reads afterwards).  A row occupies the same positions
+  // in the (row, tx) and (row, rank, tx) orders, so ctx.SS (the row start of
+  // every position, made for the ranks) makes each turn relative to its row,
+  // and the counters start at zero: one dense fill instead of a random store
+  // per row.
   void ordered_need(const uint32_t* srow, const uint32_t* sop) {
     const uint64_t n = ctx.nops;
     hipStream_t s = ctx.stream;
     HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s));
     HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
-    HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows, s));
+    HIP_TRY(launch::access_need_rel(sop, ctx.gstart.p, ctx.SS.p, n, ctx.need.p, s));
+    HIP_TRY(launch::fill_u32(ctx.counters.p, ctx.counters.n, 0, s));
   }
   void Init(int, uint64_t) override {}
   // the persistent form runs every rank level in one launch: it needs no
