@@ -58,9 +58,9 @@ hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint
 
 // --- ordered-execution preprocessing ---
 // From accesses sorted by (row, order), sop holding op_ids values: group-start
-// flags (p or 0).
+// flags (p or 0) and, when rowstart is given, row-start flags (p or 0).
 hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags,
-                              hipStream_t s);
+                              hipStream_t s, uint32_t* rowstart = nullptr);
 // need[op] = gstart[p] + pos_base for op = sop[p] & OPID_MASK;
 // counters[srow[p]] = p + pos_base at row starts.
 hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,

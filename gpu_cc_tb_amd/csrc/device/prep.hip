@@ -507,12 +507,15 @@ hipError_t radix_impl(KT* k0, uint32_t* v0, KT* k1, uint32_t* v1, uint64_t n, in
 // ---------------------------------------------------------------------------
 // Ordered-execution passes.
 // ---------------------------------------------------------------------------
-__global__ void k_access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags) {
+// rowstart (optional): p at the first access of a row, else 0
+__global__ void k_access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags,
+                                     uint32_t* rowstart) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
-    bool start = (p == 0) || srow[p] != srow[p - 1];
-    if (!start) start = ((sop[p] | sop[p - 1]) & OPID_WRITE) != 0u;
+    const bool row0 = (p == 0) || srow[p] != srow[p - 1];
+    const bool start = row0 || ((sop[p] | sop[p - 1]) & OPID_WRITE) != 0u;
     flags[p] = start ? (uint32_t)p : 0u;
+    if (rowstart) rowstart[p] = row0 ? (uint32_t)p : 0u;
   }
 }
 __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
@@ -710,8 +713,8 @@ hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint
   return scan_impl<OpMax, true>(in, out, n, tmp, s);
 }
 hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags,
-                              hipStream_t s) {
-  GC_LAUNCH(k_access_group_flags, n, srow, sop, n, flags);
+                              hipStream_t s, uint32_t* rowstart) {
+  GC_LAUNCH(k_access_group_flags, n, srow, sop, n, flags, rowstart);
 }
 hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
                        uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,

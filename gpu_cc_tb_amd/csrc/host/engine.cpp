@@ -295,13 +295,17 @@ class GaccoHost : public SchemeHost {
   // The persistent form queues every row's accesses of the whole stream in
   // transaction-id order: the outcome is that of the serial run in id order,
   // the same as batch after batch.
-  void PrepStream() override { prep_range(0, ctx.nops); }
+  void PrepStream() override { prep_range(0, ctx.nops, true); }
   void PostStream() override {}
   void PrepLaunch(uint64_t o0, uint64_t o1) override { prep_range(o0, o1); }
   bool ProbesRows() const override { return true; }
   // per-row access queues for ops [o0, o1) in (row, transaction id) order;
-  // the rows also land in rows_out[o0, o1)
-  void prep_range(uint64_t o0, uint64_t o1) {
+  // the rows also land in rows_out[o0, o1).  relative: turns count from the
+  // row's first access and the counters start from one dense zero fill (the
+  // whole stream at once); otherwise each touched row's counter is set to
+  // its first position, which costs a random store per row but no fill of
+  // the whole array (one batch at a time).
+  void prep_range(uint64_t o0, uint64_t o1, bool relative = false) {
     const uint64_t n = o1 - o0;
     if (n == 0) return;
     hipStream_t s = ctx.stream;
@@ -310,6 +314,18 @@ class GaccoHost : public SchemeHost {
     // rows are below cfg.rows; a miss (IDX_MISS) has all row bits set and sorts last
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.cfg.rows),
                                    ctx.sort_tmp.p, &srow, &sop, s));
+    if (relative) {
+      // the sort's other key and value buffers are free now: row-start flags
+      // and the row start of every position
+      uint32_t* rflags = srow == ctx.sk0.p ? ctx.sk1.p : ctx.sk0.p;
+      uint32_t* rstart = sop == ctx.sv0.p ? ctx.sv1.p : ctx.sv0.p;
+      HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s, rflags));
+      HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
+      HIP_TRY(launch::inclusive_max_u32(rflags, rstart, n, ctx.sort_tmp.p, s));
+      HIP_TRY(launch::access_need_rel(sop, ctx.gstart.p, rstart, n, ctx.need.p, s));
+      HIP_TRY(launch::fill_u32(ctx.counters.p, ctx.counters.n, 0, s));
+      return;
+    }
     HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s));
     HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
     HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows, s));
