@@ -33,7 +33,7 @@ TABLE_PATH = os.path.join(HERE, "tuned.json")
 THETAS = {"0": 0.1, "1": 0.6, "2": 0.8, "3": 0.9, "4": 0.99}
 READS = {"w": 0.5, "r": 0.9, "o": 1.0}
 
-WD = [2, 3, 4]
+WD = [1, 2, 3, 4]
 BS = [4, 8]
 # admission windows (transactions in flight; 0 = device-filling).  The
 # adaptive window (-1) is left out: the library refuses it in this build.

@@ -54,6 +54,27 @@ def test_form_parity(scheme, case, form):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(1, 4, 65536), (1, 8, 32768), (2, 4, 32768), (2, 8, 16384)])
+@pytest.mark.parametrize("scheme", list(g.SCHEMES))
+def test_persistent_narrow_groups(scheme, shape):
+    """The persistent form with 2 and 4 lanes per transaction (16 and 8 op
+    slots per lane) and large windows, the low-contention tuned shapes:
+    serial replay on variable-length transactions."""
+    wd, bs, window = shape
+    rows, ntx = 2048, 3000
+    print(f"[narrow] scheme={scheme} wd={wd} bs={bs} window={window}", flush=True)
+    w = g.generate(rows, ntx, theta=0.6, read_ratio=0.7, len_min=4, len_max=32)
+    with g.Context(scheme, rows=rows, warp_density=wd, block_size=bs, batch_size=1000,
+                   persist_window=window) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run("persistent")
+        res = ctx.fetch(aborts=True)
+    assert st["committed"] == w.ntx and st["error_flags"] == 0
+    check_replay(w, res, rows, g.SEED)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("scheme", list(g.SCHEMES))
 def test_uneven_batch_stream(scheme):
     s = wl.batched_stream(2048, 12, 16, 400, len_lo=4, len_hi=32, theta_lo=0.0, theta_hi=0.99)
