@@ -40,7 +40,7 @@ BS = [4, 8]
 WINDOWS = [0, 32768, 16384, 8192, 4096, 1024, 512, 256]
 
 # starting shapes (from sweeps of the plain form at low contention)
-DEFAULT_SHAPE = {"tpl_nw": (3, 8), "tpl_wd": (3, 8), "to": (4, 8), "mvcc": (4, 4), "silo": (3, 8),
+DEFAULT_SHAPE = {"tpl_nw": (3, 8), "tpl_wd": (3, 8), "to": (4, 8), "mvcc": (3, 8), "silo": (3, 8),
                  "tictoc": (3, 8), "gacco": (2, 8), "gputx": (3, 8)}
 
 
