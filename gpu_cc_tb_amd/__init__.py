@@ -107,6 +107,7 @@ def load_library() -> C.CDLL:
         "gcctb_build_index": (C.c_int, [P, _u32p, _u32p, C.c_uint64]),
         "gcctb_probe": (C.c_int, [P, _u32p, C.c_uint64, _u32p]),
         "gcctb_load_ops": (C.c_int, [P, _u64p, _u64p, C.c_uint64]),
+        "gcctb_check_ops": (C.c_int, [_u64p, _u64p, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64)]),
         "gcctb_run_plain": (C.c_int, [P, C.POINTER(Stats)]),
         "gcctb_run_graph": (C.c_int, [P, C.POINTER(Stats)]),
         "gcctb_run_persistent": (C.c_int, [P, C.POINTER(Stats)]),
@@ -173,6 +174,18 @@ def generate(rows: int, ntx: int, *, theta: float, read_ratio: float, len_min: i
     n = C.c_uint64(0)
     _check(lib.gcctb_generate(C.byref(w), ops, off, C.byref(n)))
     return Ops(ops[: n.value].copy() if n.value else np.zeros(0, dtype=np.uint64), off)
+
+
+def check_ops(w: Ops, cols: int = 10) -> int:
+    """The host-side input checks of load_ops (distinct keys per transaction,
+    at most 32 ops, columns in range); returns the read count or raises."""
+    ops = np.ascontiguousarray(w.ops, dtype=np.uint64)
+    if ops.size == 0:
+        ops = np.zeros(1, dtype=np.uint64)
+    n = C.c_uint64(0)
+    _check(load_library().gcctb_check_ops(ops, np.ascontiguousarray(w.tx_off, dtype=np.uint64), w.ntx, cols,
+                                          C.byref(n)))
+    return n.value
 
 
 def zeta(n: int, theta: float) -> float:

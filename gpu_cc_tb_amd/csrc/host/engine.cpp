@@ -92,6 +92,9 @@ struct gcctb_ctx {
   uint32_t poll_ticks = 25;  // 0.25 us (measured sweep 6..100 at theta 0.99)
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
+  // pinned host word: the error flags of the launches since the last metrics
+  // clear, copied behind each launch so the host sees them at its sync
+  uint64_t* h_err = nullptr;
   // table + index
   gcctb::DevBuf<uint64_t> cells;
   gcctb::DevBuf<uint32_t> buckets;  // hash index
@@ -247,7 +250,8 @@ class TwoPLHost : public SchemeHost {
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
     HIP_TRY(launch::exec_tpl(wait_die, ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
   }
-  // Locks are free again once every transaction of the batch committed.
+  // Locks are free again once every transaction of the batch committed; a
+  // launch that stopped on an error is followed by clear_cc_state instead.
   void Post(int, uint64_t, uint64_t) override {}
   bool wait_die;
 };
@@ -467,6 +471,14 @@ int hip_fail(const HipFailure& f) {
 void build_index(gcctb_ctx& c, const uint32_t* dkeys, const uint32_t* dvals, uint64_t n) {
   const uint64_t nb = idx_buckets_for(n);
   if (nb > 0xffffffffull) throw HipFailure{hipErrorInvalidValue, "index too large"};
+  // A captured graph holds the old bucket array in its kernels' arguments:
+  // it is rebuilt on its next launch, and dropped now, before the buckets it
+  // points to are freed.
+  c.inputs_gen++;
+  if (c.graph_exec) {
+    (void)hipGraphExecDestroy(c.graph_exec);
+    c.graph_exec = nullptr;
+  }
   c.ix = IndexView{};
   c.buckets.release();
   c.buckets.alloc(nb * 2 * IDX_SLOTS);
@@ -476,6 +488,25 @@ void build_index(gcctb_ctx& c, const uint32_t* dkeys, const uint32_t* dvals, uin
   c.ix.buckets = c.buckets.p;
   c.ix.nbuckets = (uint32_t)nb;
   c.ix.length = (uint32_t)n;
+}
+
+// Copy the error word (the kernels OR their flags into metrics slot 0) into
+// pinned host memory behind the work queued so far.
+void enqueue_error_copy(gcctb_ctx& c) {
+  HIP_TRY(hipMemcpyAsync(c.h_err, &c.metrics.p[0].error_flags, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
+}
+
+// After a launch that reported an error (a watchdog stop above all), CC words
+// can still hold the locks or marks of transactions that never finished: 2PL
+// releases its locks only at commit, so its per-batch Post has nothing to
+// restore after a clean batch.  Every CC word, chain head and turn counter is
+// cleared, which is the state outside a run.
+void clear_cc_state(gcctb_ctx& c) {
+  const uint64_t rows = c.cfg.rows;
+  HIP_TRY(launch::fill_u64(c.cc.p, rows, 0, c.stream));
+  if (c.vprev.p) HIP_TRY(launch::fill_u32(c.vprev.p, rows, 0, c.stream));
+  if (c.counters.p) HIP_TRY(launch::fill_u32(c.counters.p, rows, 0, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
 }
 
 void ensure_state(gcctb_ctx& c) {
@@ -558,6 +589,8 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
     HIP_TRY(hipSetDevice(0));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipHostMalloc((void**)&c->h_err, sizeof(uint64_t), hipHostMallocDefault));
+    *c->h_err = 0;
     switch (c->scheme) {
       case SCHEME_TPL_NW: c->host.reset(new TwoPLHost(*c, false)); break;
       case SCHEME_TPL_WD: c->host.reset(new TwoPLHost(*c, true)); break;
@@ -588,6 +621,7 @@ void gcctb_destroy(gcctb_ctx* c) {
   c->host.reset();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->h_err) (void)hipHostFree(c->h_err);
   hipStream_t s = c->stream;
   delete c;  // DevBuf destructors free device memory
   if (s) (void)hipStreamDestroy(s);
@@ -618,6 +652,15 @@ int gcctb_build_index(gcctb_ctx* c, const uint32_t* keys, const uint32_t* vals, 
   for (uint64_t i = 0; i < n; ++i) {
     k[i] = kv[i].first;
     v[i] = kv[i].second;
+  }
+  // The kernels assume that the ops of one transaction touch distinct rows;
+  // load_ops checks that their keys are distinct, so the index must map
+  // distinct keys to distinct rows (a primary-key index, key -> row id).
+  {
+    std::vector<uint32_t> sv(v);
+    std::sort(sv.begin(), sv.end());
+    for (uint64_t i = 1; i < n; ++i)
+      if (sv[i] == sv[i - 1]) return set_error(GCCTB_EINVAL, "two index keys map to one row");
   }
   try {
     DevBuf<uint32_t> dk, dv;
@@ -685,8 +728,8 @@ int gcctb_seed_table(gcctb_ctx* c) {
   return GCCTB_OK;
 }
 
-int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx) {
-  if (!c || !tx_off || (!ops && ntx)) return set_error(GCCTB_EINVAL, "null argument");
+int gcctb_check_ops(const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, uint32_t cols, uint64_t* nreads_out) {
+  if (!tx_off || (!ops && ntx)) return set_error(GCCTB_EINVAL, "null argument");
   if (ntx >= (1ull << 31)) return set_error(GCCTB_EINVAL, "at most 2^31-1 transactions (31-bit lock holder ids)");
   if (tx_off[0] != 0) return set_error(GCCTB_EINVAL, "tx_off[0] must be 0");
   uint64_t nreads = 0;
@@ -694,15 +737,35 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
     if (tx_off[t + 1] < tx_off[t]) return set_error(GCCTB_EINVAL, "tx_off must be non-decreasing");
     const uint64_t len = tx_off[t + 1] - tx_off[t];
     if (len > (uint64_t)MAX_OPS) return set_error(GCCTB_EINVAL, "a transaction has more than 32 ops");
+    // Each op of a transaction must name its own row: the kernels hold one
+    // lock, mark or turn per op, and two ops on one row would conflict with
+    // each other.  The reference's generator drops repeated keys
+    // (generation/gen_ycsb.cu:129-136); keys are checked here, and
+    // gcctb_build_index keeps distinct keys on distinct rows.
+    uint32_t keys[MAX_OPS];
     for (uint64_t i = tx_off[t]; i < tx_off[t + 1]; ++i) {
-      if (op_col(ops[i]) >= c->cfg.cols) return set_error(GCCTB_EINVAL, "op column out of range");
+      if (op_col(ops[i]) >= cols) return set_error(GCCTB_EINVAL, "op column out of range");
       nreads += op_is_write(ops[i]) ? 0 : 1;
+      const uint32_t key = op_key(ops[i]);
+      const uint64_t j = i - tx_off[t];
+      for (uint64_t q = 0; q < j; ++q)
+        if (keys[q] == key)
+          return set_error(GCCTB_EINVAL, "transaction " + std::to_string(t) + " names key " + std::to_string(key) +
+                                             " twice (keys must be distinct within a transaction)");
+      keys[j] = key;
     }
   }
-  const uint64_t nops = tx_off[ntx];
   // op ids carry a write flag in bit 31 through the GaccO / GPUTx sorts
-  if (nops >= (1ull << 31)) return set_error(GCCTB_EINVAL, "at most 2^31-1 ops");
-  // MVCC version pool: the largest number of writes in one batch
+  if (tx_off[ntx] >= (1ull << 31)) return set_error(GCCTB_EINVAL, "at most 2^31-1 ops");
+  if (nreads_out) *nreads_out = nreads;
+  return GCCTB_OK;
+}
+
+int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx) {
+  if (!c) return set_error(GCCTB_EINVAL, "null argument");
+  uint64_t nreads = 0;
+  if (const int rc = gcctb_check_ops(ops, tx_off, ntx, c->cfg.cols, &nreads)) return rc;
+  const uint64_t nops = tx_off[ntx];
   uint64_t cap = 0;
   // MVCC version pool: one node per write of the whole load (the persistent
   // form keeps every version of the stream; a plain batch needs at most that)
@@ -824,6 +887,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     hipStream_t s = c->stream;
     HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics) * METRIC_SLOTS, s));
     HIP_TRY(hipStreamSynchronize(s));
+    *c->h_err = 0;
     float ms = 0;
     double pre = 0, proc = 0;
     uint64_t launches = 0;
@@ -844,6 +908,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       c->rows_probed = false;
       HIP_TRY(hipEventRecord(c->ev[4], s));
       c->host->PostStream();
+      enqueue_error_copy(*c);
       HIP_TRY(hipEventRecord(c->ev[5], s));
       HIP_TRY(hipEventSynchronize(c->ev[5]));
       HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
@@ -910,6 +975,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       c->host->Launch((int)i, batch_st, batch_st + n, false);
       HIP_TRY(hipEventRecord(c->ev[4], s));
       c->host->Post((int)i, batch_st, n);
+      enqueue_error_copy(*c);
       HIP_TRY(hipEventRecord(c->ev[5], s));
       HIP_TRY(hipEventSynchronize(c->ev[5]));
       HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
@@ -920,6 +986,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       proc += ms;
       batch_st += n;
       launches++;
+      if (*c->h_err) break;  // the first batch that reported an error ends the run
     }
     }
     HIP_TRY(hipEventRecord(c->ev[6], s));
@@ -931,6 +998,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     st.exec_kernel_ms = proc;
     st.batches = launches;
     m = collect_metrics(c);
+    if (m.error_flags) clear_cc_state(*c);
     if (c->scheme == SCHEME_GPUTX) {
       st.ranks = c->nranks;
       if (c->nops) HIP_TRY(hipMemcpy(&st.same_rank_conflicts, c->conflicts.p, 8, hipMemcpyDeviceToHost));
@@ -1057,6 +1125,7 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
     }
     HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics) * METRIC_SLOTS, s));
     HIP_TRY(hipStreamSynchronize(s));
+    *c->h_err = 0;
     if (c->scheme == SCHEME_MVCC) c->pool_cap = std::min(c->pool_cap, pool_nodes);
     std::vector<double> lat(c->ntx, 0.0);
     c->tcommit.alloc(c->ntx);
@@ -1103,6 +1172,7 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
       c->host->Launch((int)launches, lo, hi, true);
       c->rows_probed = false;
       c->host->Post((int)launches, lo, hi - lo);
+      enqueue_error_copy(*c);
       HIP_TRY(hipEventRecord(c->ev[7], s));
       HIP_TRY(hipEventSynchronize(c->ev[7]));
       const auto td = clk::now();
@@ -1112,6 +1182,7 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
       launch_txs += (double)txs;
       ++launches;
       next = end;
+      if (*c->h_err) break;  // a launch that reported an error ends the loop
     }
     st.tot_ms = since(clk::now());
     c->record_commit = false;
@@ -1138,6 +1209,7 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
       }
     }
     const Metrics m = collect_metrics(c);
+    if (m.error_flags) clear_cc_state(*c);
     st.committed = m.committed;
     st.aborts = m.abort;
     st.error_flags = m.error_flags;

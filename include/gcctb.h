@@ -92,7 +92,10 @@ int gcctb_seed_table(gcctb_ctx* ctx);
  * array.  Called by load_table/seed_table. */
 int gcctb_build_index_identity(gcctb_ctx* ctx);
 /* Build the index from host (key, value) pairs (any order; keys distinct,
- * keys < 2^32-1, values < rows). */
+ * keys < 2^32-1, values < rows and distinct: a primary-key index, so that
+ * distinct keys of a transaction name distinct rows; GCCTB_EINVAL otherwise).
+ * A graph captured by gcctb_run_graph is dropped and captured again on its
+ * next call. */
 int gcctb_build_index(gcctb_ctx* ctx, const uint32_t* keys, const uint32_t* vals, uint64_t n);
 /* Index probe for tests: rows_out[i] = Find(keys[i]) (include/index.cuh:74-91);
  * a miss returns the index length. */
@@ -101,14 +104,24 @@ int gcctb_probe(gcctb_ctx* ctx, const uint32_t* keys, uint64_t n, uint32_t* rows
 /* Upload transactions in CSR form: ops[tx_off[t] .. tx_off[t+1]) are the ops
  * of transaction t (encoding: key bits 0..31, column bits 32..39, write bit
  * 40).  Replaces the .txs load, flattening and DynamicTransactionSet_CPU
- * (benchmark_cpu/ycsb.cu:116-157, include/transaction.cuh:199-368). */
+ * (benchmark_cpu/ycsb.cu:116-157, include/transaction.cuh:199-368).
+ * Contract: at most 32 ops per transaction, and the keys of one transaction
+ * are distinct (the reference's generator drops repeated keys,
+ * generation/gen_ycsb.cu:129-136); a repeated key returns GCCTB_EINVAL. */
 int gcctb_load_ops(gcctb_ctx* ctx, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx);
+/* The input checks of gcctb_load_ops on the host alone (no context, no GPU):
+ * offsets, lengths, columns below cols, distinct keys per transaction, and
+ * the op and transaction limits.  *nreads (may be null) gets the read count. */
+int gcctb_check_ops(const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, uint32_t cols, uint64_t* nreads);
 
 /* Run every loaded transaction with the reference's launch structure: Init0
  * splits into batches, then per batch Init (reset / preprocess) + one kernel
  * launch + synchronization (benchmark_cpu/ycsb.cu:249-291, include/cc.cuh:29-36,
  * benchmark/ycsb.cu:21-111).  The table keeps the result; run again to
- * continue on it. */
+ * continue on it.  A batch whose kernel reports an error (GCCTB_EKERNEL, e.g.
+ * the per-wave watchdog, GCCTB_WATCHDOG_S seconds) ends the run; every CC
+ * word is then cleared, so no lock or mark of an unfinished transaction
+ * outlives the call (the same holds for every run and serve entry point). */
 int gcctb_run_plain(gcctb_ctx* ctx, gcctb_stats* out);
 
 /* The same kernels with the per-batch sequence captured once into a hipGraph

@@ -44,6 +44,11 @@ uint64_t orc_find(const uint32_t* sorted_keys, const uint64_t* vals, uint32_t le
  * allocation failure. */
 int64_t orc_replay(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
                    const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads);
+/* orc_replay with the work split over `threads` threads by blocks of rows
+ * (a read depends only on earlier writes to its own row): same table, reads
+ * and tie count. */
+int64_t orc_replay_mt(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                      const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads, int threads);
 /* Replay in plain transaction-id order (GaccO's serial order inside a batch). */
 void orc_replay_txid(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
                      const uint64_t* tx_off, uint64_t ntx, uint64_t* reads);
@@ -54,15 +59,6 @@ void orc_replay_txid(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* tabl
  * skipped. */
 int orc_gputx_rank(uint64_t rows, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx,
                    uint32_t* rank, uint32_t* counts, uint64_t* nranks);
-
-/* ---- conflict-graph serializability check (src/runtime.cu:23-98) ----
- * Given the committed serial order and the read values, checks that every read
- * returned the value of the latest preceding write in that order (the
- * "reads-from" relation) -- equivalent to the replay check but reported per
- * op.  Returns the number of mismatching reads. */
-uint64_t orc_check_reads(uint64_t rows, uint32_t cols, uint64_t seed, const uint64_t* init_table,
-                         const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
-                         const uint64_t* reads);
 
 /* ---- multi-threaded CPU path (the CPU baseline): the same protocols on C11
  * atomics, one transaction per thread at a time, ops in key order.

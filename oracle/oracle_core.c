@@ -3,6 +3,7 @@
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -181,24 +182,89 @@ void orc_replay_txid(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* tabl
   for (uint64_t t = 0; t < ntx; ++t) apply_tx(rows, cols, seed, table, ops, tx_off, t, reads);
 }
 
-uint64_t orc_check_reads(uint64_t rows, uint32_t cols, uint64_t seed, const uint64_t* init_table,
-                         const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
-                         const uint64_t* reads) {
-  uint64_t* table = (uint64_t*)malloc(sizeof(uint64_t) * rows * cols);
-  uint64_t* exp = (uint64_t*)malloc(sizeof(uint64_t) * (tx_off[ntx] ? tx_off[ntx] : 1));
-  if (!table || !exp) {
-    free(table);
-    free(exp);
-    return ~0ull;
+/* Serial replay split over threads by row.  A read returns the latest
+ * preceding write to its own row in the order, so the ops of different rows
+ * never influence each other: thread k walks every transaction in the same
+ * sorted order and applies only the ops whose row lies in its block of rows.
+ * The table and the reads equal orc_replay's (tests compare the two). */
+typedef struct {
+  uint64_t rows, row_lo, row_hi, ntx, seed;
+  uint32_t cols;
+  uint64_t* table;
+  const uint64_t* ops;
+  const uint64_t* tx_off;
+  const okey_t* ord;
+  uint64_t* reads;
+} replay_part_t;
+static void* replay_part(void* arg) {
+  const replay_part_t* p = (const replay_part_t*)arg;
+  for (uint64_t k = 0; k < p->ntx; ++k) {
+    const uint64_t t = p->ord[k].tx;
+    for (uint64_t i = p->tx_off[t]; i < p->tx_off[t + 1]; ++i) {
+      const uint64_t op = p->ops[i];
+      const uint64_t row = (uint32_t)op;
+      if (row < p->row_lo || row >= p->row_hi) continue;
+      const uint32_t col = (uint32_t)(op >> 32) & 0xff;
+      if (row >= p->rows || col >= p->cols) {
+        if (p->reads) p->reads[i] = 0;
+        continue;
+      }
+      uint64_t* cell = p->table + row * p->cols + col;
+      if ((op >> 40) & 1) {
+        *cell = orc_write_value(p->seed, t, (uint32_t)(i - p->tx_off[t]));
+        if (p->reads) p->reads[i] = 0;
+      } else if (p->reads) {
+        p->reads[i] = *cell;
+      }
+    }
   }
-  memcpy(table, init_table, sizeof(uint64_t) * rows * cols);
-  orc_replay(rows, cols, seed, table, ops, tx_off, ntx, order, exp);
-  uint64_t bad = 0;
-  for (uint64_t i = 0; i < tx_off[ntx]; ++i)
-    if (!((ops[i] >> 40) & 1) && exp[i] != reads[i]) bad++;
-  free(table);
-  free(exp);
-  return bad;
+  return NULL;
+}
+
+int64_t orc_replay_mt(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                      const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  okey_t* ord = (okey_t*)malloc(sizeof(okey_t) * (ntx ? ntx : 1));
+  if (!ord) return -1;
+  for (uint64_t t = 0; t < ntx; ++t) {
+    ord[t].b = order[3 * t];
+    ord[t].k0 = order[3 * t + 1];
+    ord[t].k1 = order[3 * t + 2];
+    ord[t].tx = t;
+  }
+  qsort(ord, ntx, sizeof(okey_t), cmp_okey);
+  int64_t ties = 0;
+  for (uint64_t i = 1; i < ntx; ++i)
+    if (ord[i].b == ord[i - 1].b && ord[i].k0 == ord[i - 1].k0 && ord[i].k1 == ord[i - 1].k1) ties++;
+  /* ops whose key is not a row go to the last part, which clears their reads */
+  const uint64_t span = (rows + (uint64_t)threads - 1) / (uint64_t)threads;
+  pthread_t th[256];
+  replay_part_t part[256];
+  int started = 0;
+  for (int k = 0; k < threads; ++k) {
+    part[k].rows = rows;
+    part[k].row_lo = (uint64_t)k * span;
+    part[k].row_hi = k == threads - 1 ? ~0ull : (uint64_t)(k + 1) * span;
+    part[k].ntx = ntx;
+    part[k].seed = seed;
+    part[k].cols = cols;
+    part[k].table = table;
+    part[k].ops = ops;
+    part[k].tx_off = tx_off;
+    part[k].ord = ord;
+    part[k].reads = reads;
+  }
+  for (int k = 1; k < threads; ++k) {
+    if (pthread_create(&th[k], NULL, replay_part, &part[k]) != 0) break;
+    started = k;
+  }
+  /* threads that could not be started run here, after part 0 */
+  replay_part(&part[0]);
+  for (int k = started + 1; k < threads; ++k) replay_part(&part[k]);
+  for (int k = 1; k <= started; ++k) pthread_join(th[k], NULL);
+  free(ord);
+  return ties;
 }
 
 /* ---- GPUTx one-pass rank ----
@@ -209,47 +275,47 @@ uint64_t orc_check_reads(uint64_t rows, uint32_t cols, uint64_t seed, const uint
  * past the scanned offsets; here every segment ends where the next row starts
  * (the last one at the op count).  counts = histogram of ranks
  * (src/gputx.cu:30-38); nranks = max local rank + 1 (gputx.cuh:283-294). */
-typedef struct {
-  uint32_t row, tx;
-  uint8_t write;
-} acc_t;
-static int cmp_acc(const void* pa, const void* pb) {
-  const acc_t* a = (const acc_t*)pa;
-  const acc_t* b = (const acc_t*)pb;
-  if (a->row != b->row) return a->row < b->row ? -1 : 1;
-  if (a->tx != b->tx) return a->tx < b->tx ? -1 : 1;
-  return 0;
-}
 int orc_gputx_rank(uint64_t rows, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx,
                    uint32_t* rank, uint32_t* counts, uint64_t* nranks) {
+  /* accesses in (row, tx) order by a counting sort over rows: transactions
+   * are visited in id order, so each row's list is in ascending tx */
   const uint64_t nops = tx_off[ntx];
-  acc_t* acc = (acc_t*)malloc(sizeof(acc_t) * (nops ? nops : 1));
-  if (!acc) return -1;
-  uint64_t n = 0;
+  uint64_t* start = (uint64_t*)calloc(rows + 1, sizeof(uint64_t));
+  uint32_t* acc = (uint32_t*)malloc(sizeof(uint32_t) * (nops ? nops : 1)); /* tx | write << 31 */
+  if (!start || !acc) {
+    free(start);
+    free(acc);
+    return -1;
+  }
+  for (uint64_t i = 0; i < nops; ++i) {
+    const uint32_t row = (uint32_t)ops[i];
+    if (row < rows) start[row + 1]++;
+  }
+  for (uint64_t r = 0; r < rows; ++r) start[r + 1] += start[r];
+  const uint64_t n = start[rows];
   for (uint64_t t = 0; t < ntx; ++t)
     for (uint64_t i = tx_off[t]; i < tx_off[t + 1]; ++i) {
       const uint32_t row = (uint32_t)ops[i];
       if (row >= rows) continue;
-      acc[n].row = row;
-      acc[n].tx = (uint32_t)t;
-      acc[n].write = (uint8_t)((ops[i] >> 40) & 1);
-      n++;
+      acc[start[row]++] = (uint32_t)t | ((uint32_t)((ops[i] >> 40) & 1) << 31);
     }
-  qsort(acc, n, sizeof(acc_t), cmp_acc);
+  /* start[r] now holds the end of row r's list, which is row r+1's start */
   for (uint64_t t = 0; t < ntx; ++t) rank[t] = 0;
   uint32_t tot = 0;
   uint64_t st = 0;
-  while (st < n) {
-    uint64_t en = st;
-    while (en < n && acc[en].row == acc[st].row) en++;
+  for (uint64_t r = 0; r < rows; ++r) {
+    const uint64_t en = start[r];
+    if (en == st) continue;
     uint32_t cur = 0;
     for (uint64_t i = st; i < en; ++i) {
-      if (rank[acc[i].tx] < cur) rank[acc[i].tx] = cur;
-      if (i + 1 < en && (acc[i].write || acc[i + 1].write)) cur++;
+      const uint32_t tx = acc[i] & 0x7fffffffu;
+      if (rank[tx] < cur) rank[tx] = cur;
+      if (i + 1 < en && ((acc[i] | acc[i + 1]) >> 31)) cur++;
     }
     if (cur + 1 > tot) tot = cur + 1;
     st = en;
   }
+  free(start);
   free(acc);
   if (n == 0 && ntx) tot = 1;
   if (nranks) *nranks = tot;

@@ -47,15 +47,15 @@ def lib() -> C.CDLL:
         L.orc_replay.restype = C.c_int64
         L.orc_replay.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64, _u64p,
                                  _u64p]
+        L.orc_replay_mt.restype = C.c_int64
+        L.orc_replay_mt.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64, _u64p,
+                                    _u64p, C.c_int]
         L.orc_replay_txid.restype = None
         L.orc_replay_txid.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64,
                                       _u64p]
         L.orc_gputx_rank.restype = C.c_int
         L.orc_gputx_rank.argtypes = [C.c_uint64, _u64p, _u64p, C.c_uint64, _u32p, _u32p,
                                      C.POINTER(C.c_uint64)]
-        L.orc_check_reads.restype = C.c_uint64
-        L.orc_check_reads.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64,
-                                      _u64p, _u64p]
         L.orc_cpu_run.restype = C.c_int
         L.orc_cpu_run.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64,
                                   C.c_uint64, C.c_int, _u64p, _u64p, C.POINTER(CpuStats)]
@@ -74,12 +74,19 @@ def seed_table(seed: int, rows: int, cols: int) -> np.ndarray:
     return t
 
 
-def replay(rows, cols, seed, table, ops, tx_off, order):
-    """Serial replay in (batch, key0, key1) order -> (final table, reads, ties)."""
-    t = np.array(table, dtype=np.uint64, copy=True)
+def replay(rows, cols, seed, table, ops, tx_off, order, threads=1, in_place=False):
+    """Serial replay in (batch, key0, key1) order -> (final table, reads, ties).
+    threads > 1 splits the replay over threads by blocks of rows
+    (orc_replay_mt, same result); in_place updates `table` itself."""
+    t = table if in_place else np.array(table, dtype=np.uint64, copy=True)
+    assert t.dtype == np.uint64 and t.flags.c_contiguous
     reads = np.zeros(max(1, len(ops)), dtype=np.uint64)
     ntx = len(tx_off) - 1
-    ties = lib().orc_replay(rows, cols, seed, t, _a64(ops), _a64(tx_off), ntx, _a64(order), reads)
+    if threads > 1:
+        ties = lib().orc_replay_mt(rows, cols, seed, t, _a64(ops), _a64(tx_off), ntx, _a64(order), reads, threads)
+    else:
+        ties = lib().orc_replay(rows, cols, seed, t, _a64(ops), _a64(tx_off), ntx, _a64(order), reads)
+    assert ties >= 0, "oracle allocation failed"
     return t, reads[: len(ops)], ties
 
 

@@ -79,3 +79,26 @@ def test_sort_pairs_arguments():
     assert lib.gcctb_sort_pairs(kp, vp, 4, 4, 33, None) == EINVAL
     assert lib.gcctb_sort_pairs(None, vp, 4, 4, 8, None) == EINVAL
     assert lib.gcctb_sort_pairs(kp, vp, 0, 4, 8, None) == 0
+
+
+def test_check_ops_rejects_repeated_keys():
+    """load_ops' host checks (gcctb_check_ops): a transaction that names one
+    key twice, a column past the table, or more than 32 ops is refused with
+    GCCTB_EINVAL before anything reaches the device."""
+    w = g.generate(1000, 50, theta=0.5, read_ratio=0.5, len_min=8, len_max=8)
+    assert g.check_ops(w) == int(np.count_nonzero(~g.op_is_write(w.ops)))
+    dup = w.ops.copy()
+    dup[8 * 7 + 3] = g.make_ops([int(g.op_key(dup[8 * 7 + 5:8 * 7 + 6])[0])], [0], [False])[0]
+    with pytest.raises(g.GcctbError) as e:
+        g.check_ops(g.Ops(dup, w.tx_off))
+    assert e.value.code == EINVAL and "twice" in str(e.value) and "transaction 7" in str(e.value)
+    with pytest.raises(g.GcctbError) as e:
+        g.check_ops(w, cols=int(g.op_col(w.ops).max()))
+    assert e.value.code == EINVAL
+    ops = g.make_ops(np.arange(33), np.zeros(33), np.zeros(33, dtype=bool))
+    with pytest.raises(g.GcctbError) as e:
+        g.check_ops(g.Ops(ops, np.array([0, 33], dtype=np.uint64)))
+    assert e.value.code == EINVAL
+    # the same key in two different transactions is fine
+    two = g.make_ops([5, 5], [1, 2], [True, False])
+    assert g.check_ops(g.Ops(two, np.array([0, 1, 2], dtype=np.uint64))) == 1

@@ -224,6 +224,60 @@ def test_gputx_rank_golden(case):
     assert list(counts) == case["counts"]
 
 
+def _rank_by_definition(rows, ops, off):
+    """src/gputx.cu:17-27 restated with Python loops (small inputs only):
+    accesses of a row in transaction-id order, the local rank counts the
+    neighbour pairs before an access of which one writes."""
+    ntx = len(off) - 1
+    per_row = {}
+    for t in range(ntx):
+        for i in range(int(off[t]), int(off[t + 1])):
+            r = int(ops[i]) & 0xFFFFFFFF
+            if r < rows:
+                per_row.setdefault(r, []).append((t, bool((int(ops[i]) >> 40) & 1)))
+    rank = [0] * ntx
+    for r in sorted(per_row):
+        acc = per_row[r]
+        cur = 0
+        for i, (t, wr) in enumerate(acc):
+            rank[t] = max(rank[t], cur)
+            if i + 1 < len(acc) and (wr or acc[i + 1][1]):
+                cur += 1
+    return np.array(rank, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_gputx_rank_matches_definition(seed):
+    """The counting-sort rank equals the definition walked access by access,
+    including keys past the table (skipped) and empty transactions."""
+    rows = 40
+    w = g.generate(rows + 8, 300, theta=0.7, read_ratio=0.6, len_min=1, len_max=12, seed=seed)
+    off = w.tx_off.copy()
+    off[100:] = off[100:] - (off[100] - off[99])  # transaction 99 becomes empty
+    ops = np.delete(w.ops, np.arange(int(w.tx_off[99]), int(w.tx_off[100])))
+    rank, counts = orc.gputx_rank(rows, ops, off)
+    exp = _rank_by_definition(rows, ops, off)
+    assert np.array_equal(rank, exp)
+    assert np.array_equal(counts, np.bincount(exp, minlength=int(exp.max()) + 1))
+
+
+@pytest.mark.parametrize("threads", [2, 3, 8, 64])
+def test_replay_mt_matches_serial(threads):
+    """The row-partitioned replay gives the serial replay's table, reads and
+    tie count for any thread count (more threads than rows included), with
+    keys past the table and tied order keys in the input."""
+    rows = 50
+    w = g.generate(rows + 5, 2000, theta=0.9, read_ratio=0.5, len_min=1, len_max=20, seed=threads)
+    init = orc.seed_table(g.SEED, rows, 10)
+    order = np.random.default_rng(threads).integers(0, 40, 3 * w.ntx).astype(np.uint64)
+    order[0::3] &= np.uint64(1)  # batch 0 or 1: many tied keys
+    t1, r1, ties1 = orc.replay(rows, 10, g.SEED, init, w.ops, w.tx_off, order)
+    t2, r2, ties2 = orc.replay(rows, 10, g.SEED, init, w.ops, w.tx_off, order, threads=threads)
+    assert ties1 == ties2 > 0
+    assert np.array_equal(t1, t2)
+    assert np.array_equal(r1, r2)
+
+
 def test_gputx_rank_properties_random():
     # every pair of conflicting accesses in a row's tx order has non-decreasing
     # ranks... is NOT guaranteed (no propagation); what is: a transaction's rank
