@@ -17,6 +17,14 @@ HBM.  `value` is whole-job committed Tx/s of the batched form (8 x 1,048,576 x
 steps transactions / time).  The plain form (the reference's per batch
 reset + launch + synchronize) is timed the same way and reported beside it.
 
+Batched-vs-plain legs (one pass of each form per scheme, preparation inside
+the timed region, at full size):
+  stream:  the 256-batch stream (sizes log-uniform in [1,024, 262,144], 4-32
+           ops, per-batch theta U[0, 0.99], read ratio 0.5 / 0.9 / 1.0) on
+           10,485,760 rows: persistent form against per-batch plain launches.
+  theta99: 1,048,576 transactions of 16 ops at Zipf theta 0.99, read ratio 0.9
+           and 0.5, batch 65,536, on 10,485,760 rows: persistent over plain.
+
 Latency: the serving form on a 67,108,864-row table with requests of 16 ops,
 theta 0.9, read ratio 0.8, in batches of log-uniform size in [256, 65,536]
 arriving as a seeded Poisson process at a fraction of the measured peak.
@@ -61,9 +69,12 @@ def parse():
     p.add_argument("--serve-rows", type=int, default=67108864)
     p.add_argument("--serve-txs", type=int, default=1000000,
                    help="requests per serving run (enough launches for stable percentiles)")
-    p.add_argument("--serve-load", type=float, default=0.5, help="offered load, fraction of measured peak")
+    p.add_argument("--serve-loads", default="0.25,0.5,0.9", help="offered loads, fractions of measured peak")
     p.add_argument("--serve-schemes", default="mvcc,silo")
     p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--no-stream", action="store_true", help="skip the 256-batch stream leg")
+    p.add_argument("--no-theta99", action="store_true", help="skip the theta 0.99 leg")
+    p.add_argument("--leg-schemes", default=",".join(g.SCHEMES), help="schemes of the stream and theta 0.99 legs")
     p.add_argument("--cpu-sample", type=int, default=1048576, help="transactions per scheme for the CPU baseline")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
@@ -161,29 +172,99 @@ def run_form(ctx, form: str, steps: int):
     return tot, kern, launches, aborts
 
 
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _one_pass(scheme, rows, ops, shape, form, *, batch=65536, batch_sizes=None):
+    """One timed pass of `form` (preparation included) at the given shape."""
+    with g.Context(scheme, rows=rows, warp_density=shape["warp_density"], block_size=shape["block_size"],
+                   batch_size=batch, persist_window=shape["window"] if form == "persistent" else 0) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(ops)
+        if batch_sizes is not None:
+            ctx.set_batches(batch_sizes)
+        st = ctx.run(form)
+    return st
+
+
+def stream_leg(args, table, schemes):
+    """The 256-batch stream: persistent (tuned stream shape) against the plain
+    per-batch launches at the same shape, per scheme."""
+    t0 = time.time()
+    s = workloads.config4(rows=args.rows)
+    _progress(f"stream: {s.ntx} tx, {s.ops.nops} ops, {len(s.batch_sizes)} batches generated in {time.time() - t0:.1f} s")
+    out = {}
+    for sc in schemes:
+        shape = autotune.lookup(sc, 0.5, 0.9, table, key="stream")
+        p = _one_pass(sc, args.rows, s.ops, shape, "persistent", batch=1 << 30, batch_sizes=s.batch_sizes)
+        q = _one_pass(sc, args.rows, s.ops, shape, "plain", batch=1 << 30, batch_sizes=s.batch_sizes)
+        pv, qv = s.ntx / (p["tot_ms"] / 1e3), s.ntx / (q["tot_ms"] / 1e3)
+        out[sc] = {"persistent_txps": round(pv), "plain_txps": round(qv), "ratio": round(pv / qv, 3),
+                   "plain_launches": q["batches"], "wd": shape["warp_density"], "bs": shape["block_size"],
+                   "window": shape["window"]}
+        _progress(f"stream {sc}: persistent {pv / 1e6:.2f} M, plain {qv / 1e6:.2f} M Tx/s ({pv / qv:.2f}x)")
+    ratios = [v["ratio"] for v in out.values()]
+    return {"workload": f"256 batches log-uniform in [1024, 262144] tx, 4-32 ops, theta U[0, 0.99], "
+                        f"read ratio 0.5/0.9/1.0, {args.rows} rows", "ntx": s.ntx, "nops": s.ops.nops,
+            "per_scheme": out, "min_ratio": min(ratios) if ratios else None}
+
+
+def theta99_leg(args, table, schemes):
+    """Theta 0.99, 1,048,576 tx x 16 ops, batch 65,536: persistent over plain
+    at the tuned theta-0.99 shape, per scheme, at read ratio 0.9 and 0.5."""
+    out = {}
+    for rr in (0.9, 0.5):
+        w = g.generate(args.rows, args.txs, theta=0.99, read_ratio=rr, len_min=16)
+        res = {}
+        for sc in schemes:
+            shape = autotune.lookup(sc, 0.99, rr, table)
+            p = _one_pass(sc, args.rows, w, shape, "persistent")
+            q = _one_pass(sc, args.rows, w, shape, "plain")
+            pv, qv = w.ntx / (p["tot_ms"] / 1e3), w.ntx / (q["tot_ms"] / 1e3)
+            res[sc] = {"persistent_txps": round(pv), "plain_txps": round(qv), "ratio": round(pv / qv, 3),
+                       "persistent_aborts": p["aborts"], "plain_aborts": q["aborts"],
+                       "wd": shape["warp_density"], "bs": shape["block_size"], "window": shape["window"]}
+            _progress(f"theta99 r{rr} {sc}: persistent {pv / 1e6:.2f} M, plain {qv / 1e6:.2f} M Tx/s ({pv / qv:.2f}x)")
+        out[f"read_{rr}"] = {"per_scheme": res, "min_ratio": min(v["ratio"] for v in res.values()) if res else None}
+    return {"workload": f"{args.rows} rows, {args.txs} tx x 16 ops, theta 0.99, batch 65536", **out}
+
+
+SERVE_POOL_BYTES = 32 << 30  # the serving configuration's MVCC version-pool cap
+
+
 def serving(args, table):
-    """p50/p99 commit latency of the serving form at --serve-load of the
-    measured peak (peak = every request present at once)."""
+    """p50/p99 commit latency of the serving form (continuous batching) at
+    each --serve-loads fraction of the measured peak (peak = every request
+    present at once)."""
     # bounded sample: the first batches holding about --serve-txs requests
     s = workloads.serving_stream(args.serve_rows, 4096, 256, 65536, theta=0.9, read_ratio=0.8,
                                  max_txs=args.serve_txs)
     sizes, ops, ntx, nb = s.batch_sizes, s.ops, s.ntx, len(s.batch_sizes)
-    out = {}
+    loads = [float(x) for x in args.serve_loads.split(",") if x]
+    out = {"workload": f"{args.serve_rows} rows, {ntx} requests of 16 ops (theta 0.9, read ratio 0.8) in {nb} "
+                       f"batches of log-uniform size in [256, 65536], Poisson arrivals, MVCC pool cap 32 GB"}
     for scheme in [x for x in args.serve_schemes.split(",") if x]:
         shape = autotune.lookup(scheme, 0.9, 0.8, table)
         with g.Context(scheme, rows=args.serve_rows, warp_density=shape["warp_density"],
                        block_size=shape["block_size"], batch_size=1 << 30,
-                       persist_window=shape["window"], persist_adaptive=shape["adaptive"]) as ctx:
+                       persist_window=shape["window"]) as ctx:
             ctx.seed_table()
             ctx.load_ops(ops)
-            ctx.serve(sizes)  # warm-up
-            peak = ctx.serve(sizes)
-            arr = workloads.poisson_arrivals(sizes, args.serve_load * peak["txps"])
-            st = ctx.serve(sizes, arr, max_launch_txs=262144)
-        out[scheme] = {"load": args.serve_load, "peak_txps": round(peak["txps"]), "txps": round(st["txps"]),
-                       "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                       "max_ms": round(st["max_ms"], 3), "launches": st["launches"], "txs": ntx,
-                       "batches": nb}
+            ctx.serve(sizes, pool_cap_bytes=SERVE_POOL_BYTES)  # warm-up
+            peak = ctx.serve(sizes, pool_cap_bytes=SERVE_POOL_BYTES)
+            res = {"peak_txps": round(peak["txps"]), "loads": {}}
+            for load in loads:
+                arr = workloads.poisson_arrivals(sizes, load * peak["txps"])
+                st = ctx.serve(sizes, arr, pool_cap_bytes=SERVE_POOL_BYTES)
+                res["loads"][str(load)] = {
+                    "offered_txps": round(load * peak["txps"]), "txps": round(st["txps"]),
+                    "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                    "max_ms": round(st["max_ms"], 3), "launches": st["launches"],
+                    "admitted_live": st["admitted_live"], "deferred": st["deferred"]}
+                _progress(f"serve {scheme} load {load}: p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, "
+                          f"{st['launches']} launch(es), {st['admitted_live']} batches joined a running launch")
+        out[scheme] = res
     return out
 
 
@@ -212,7 +293,7 @@ def main():
         plain_here = not args.no_plain and (pwd, pbs) == (wd, bs)
         sp_ms = sp_kern = 0.0
         with g.Context(s, rows=args.rows, warp_density=wd, block_size=bs, batch_size=args.batch,
-                       persist_window=shape["window"], persist_adaptive=shape["adaptive"]) as ctx:
+                       persist_window=shape["window"]) as ctx:
             ctx.seed_table()
             ctx.load_ops(w)
             for _ in range(args.warmup):
@@ -229,6 +310,7 @@ def main():
                 for _ in range(args.warmup):
                     ctx.run("plain")
                 sp_ms, sp_kern, _, _ = run_form(ctx, "plain", args.steps)
+        _progress(f"bench workload {s}: batched {args.txs * args.steps / (sb_ms / 1e3) / 1e6:.1f} M Tx/s")
         per[s] = {
             "batched_txps": round(args.txs * args.steps / (sb_ms / 1e3)),
             "plain_txps": round(args.txs * args.steps / (sp_ms / 1e3)) if sp_ms else None,
@@ -293,6 +375,11 @@ def main():
         "device": info["name"],
         "gen_s": round(gen_s, 2),
     }
+    leg_schemes = [x for x in args.leg_schemes.split(",") if x]
+    if not args.no_stream:
+        line["stream"] = stream_leg(args, table, leg_schemes)
+    if not args.no_theta99:
+        line["theta99"] = theta99_leg(args, table, leg_schemes)
     if not args.no_serve:
         line["latency"] = serving(args, table)
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(w, args, schemes)

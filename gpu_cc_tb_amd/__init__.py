@@ -40,6 +40,8 @@ class Config(C.Structure):
         ("backoff_ticks", C.c_uint32),
         ("persist_window", C.c_uint32),
         ("persist_adaptive", C.c_int32),
+        ("row_ops", C.c_int32),
+        ("debug_events", C.c_int32),
     ]
 
 
@@ -68,7 +70,8 @@ class ServeStats(C.Structure):
         ("txs", C.c_uint64), ("committed", C.c_uint64), ("aborts", C.c_uint64), ("launches", C.c_uint64),
         ("deferred", C.c_uint64), ("error_flags", C.c_uint64), ("tot_ms", C.c_double), ("busy_ms", C.c_double),
         ("txps", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
-        ("mean_ms", C.c_double), ("mean_launch_txs", C.c_double),
+        ("mean_ms", C.c_double), ("mean_launch_txs", C.c_double), ("admitted_live", C.c_uint64),
+        ("pool_nodes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -81,6 +84,10 @@ class Workload(C.Structure):
         ("len_min", C.c_uint32), ("len_max", C.c_uint32), ("theta", C.c_double),
         ("read_ratio", C.c_double), ("seed", C.c_uint64), ("threads", C.c_int32),
     ]
+
+
+# gcctb_event: a read (0) / write (1) of object oid by transaction tid, or its commit (2)
+EVENT_DTYPE = np.dtype([("id", "<u8"), ("oid", "<u8"), ("tid", "<i4"), ("type", "<i4")])
 
 
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
@@ -126,6 +133,9 @@ def load_library() -> C.CDLL:
         "gcctb_microbench": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]),
         "gcctb_sort_pairs": (C.c_int, [P, P, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+        "gcctb_conflict_check": (C.c_int, [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), P, C.c_uint64,
+                                           C.POINTER(C.c_uint64), C.c_char_p, C.c_uint64]),
+        "gcctb_fetch_events": (C.c_int, [P, P, C.c_uint64, C.POINTER(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -235,7 +245,9 @@ def device_info() -> dict:
 
 
 MICROBENCH = {"cas_random": 0, "add_random": 1, "add_same": 2, "load_coherent": 3, "load_cached": 4, "copy": 5,
-              "load_pair_one_line": 6, "load_pair_two_lines": 7, "ts_read_one_line": 8, "ts_read_two_lines": 9}
+              "load_pair_one_line": 6, "load_pair_two_lines": 7, "ts_read_one_line": 8, "ts_read_two_lines": 9,
+              "add_hot16_one_line": 10, "add_hot16_16_lines": 11, "add_polled_same_line": 12,
+              "add_polled_other_lines": 13}
 
 
 def microbench(kind: str, n_words: int, threads: int = 1 << 20, iters: int = 16) -> dict:
@@ -274,12 +286,14 @@ class Context:
 
     def __init__(self, cc_scheme: str, *, rows: int, cols: int = 10, warp_density: int = 5,
                  block_size: int = 32, batch_size: int = 65536, seed: int = SEED,
-                 backoff_ticks: int = 20, persist_window: int = 0, persist_adaptive: bool = False):
+                 backoff_ticks: int = 20, persist_window: int = 0, persist_adaptive: bool = False,
+                 row_ops: bool = False, debug_events: bool = False):
         self._lib = load_library()
         self.scheme = cc_scheme
-        self.rows, self.cols, self.seed = rows, cols, seed
+        self.rows, self.cols, self.seed, self.row_ops = rows, cols, seed, row_ops
         cfg = Config(cc_scheme.encode(), warp_density, block_size, batch_size, rows, cols, seed,
-                     backoff_ticks, persist_window, 1 if persist_adaptive else 0)
+                     backoff_ticks, persist_window, 1 if persist_adaptive else 0, 1 if row_ops else 0,
+                     1 if debug_events else 0)
         h = C.c_void_p()
         _check(self._lib.gcctb_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -384,6 +398,13 @@ class Context:
             "aborts": None if a is None else a[: self.ntx],
         }
 
+    def fetch_events(self) -> np.ndarray:
+        """Event log of the last plain / graph run (debug_events contexts)."""
+        out = np.zeros(max(1, self.nops + self.ntx), dtype=EVENT_DTYPE)
+        n = C.c_uint64(0)
+        _check(self._lib.gcctb_fetch_events(self._h, out.ctypes.data_as(C.c_void_p), out.size, C.byref(n)))
+        return out[: n.value]
+
     def fetch_ranks(self):
         rank = np.zeros(max(1, self.ntx), dtype=np.uint32)
         n = C.c_uint64(0)
@@ -403,3 +424,18 @@ class Context:
         w = np.zeros(self.rows, dtype=np.uint64)
         _check(self._lib.gcctb_fetch_cc(self._h, w))
         return w
+
+
+def conflict_check(events: np.ndarray, ntx: int, report_len: int = 1 << 20) -> dict:
+    """gcctb_conflict_check on the host: {"uncommitted", "cycle" (transaction
+    ids of the first cycle found, empty when acyclic), "report" (the text the
+    reference's debug check prints)}."""
+    lib = load_library()
+    ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+    unc, clen = C.c_uint64(0), C.c_uint64(0)
+    cyc = np.zeros(max(1, ntx), dtype=np.uint64)
+    rep = C.create_string_buffer(report_len)
+    _check(lib.gcctb_conflict_check(ev.ctypes.data_as(C.c_void_p) if ev.size else None, ev.size, ntx, C.byref(unc),
+                                    cyc.ctypes.data_as(C.c_void_p), cyc.size, C.byref(clen), rep, report_len))
+    return {"uncommitted": unc.value, "cycle": cyc[: min(clen.value, cyc.size)].tolist(),
+            "report": rep.value.decode()}

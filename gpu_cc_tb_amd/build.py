@@ -36,6 +36,9 @@ DEVICE_SRCS = [
     "device/exec_silo.hip",
     "device/exec_tictoc.hip",
     "device/exec_ordered.hip",
+    "device/exec_row.hip",
+    "device/exec_evt.hip",
+    "device/exec_row_evt.hip",
     "device/microbench.hip",
 ]
 HOST_SRCS = ["host/engine.cpp", "host/workload.cpp", "host/check.cpp"]

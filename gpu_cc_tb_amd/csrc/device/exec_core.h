@@ -21,8 +21,35 @@
 #pragma once
 #include "dev_util.h"
 
+// Two variants are separate sets of kernels, so that the column-mode kernels
+// of namespace dev carry none of their code: whole-row mode (every op moves
+// its whole row, the reference's Item copies, benchmark/ycsb.cu:57,86-91) and
+// the debug event log (the reference's AddEvent, include/runtime.cuh:64-81).
+// A translation unit that defines GCCTB_ROW_OPS and/or GCCTB_EVENTS to 1
+// compiles this file into namespace dev_row, dev_evt or dev_row_evt.
+#ifndef GCCTB_ROW_OPS
+#define GCCTB_ROW_OPS 0
+#endif
+#ifndef GCCTB_EVENTS
+#define GCCTB_EVENTS 0
+#endif
+
 namespace gcctb {
+#if GCCTB_ROW_OPS && GCCTB_EVENTS
+namespace dev_row_evt {
+using namespace dev;
+#elif GCCTB_ROW_OPS
+namespace dev_row {
+using namespace dev;
+#elif GCCTB_EVENTS
+namespace dev_evt {
+using namespace dev;
+#else
 namespace dev {
+#endif
+
+constexpr bool kRowOps = GCCTB_ROW_OPS != 0;
+constexpr bool kEvents = GCCTB_EVENTS != 0;
 
 // per-op slot flags
 constexpr uint32_t M_VALID = 1u;   // op exists and its key was found in the index
@@ -65,9 +92,56 @@ __device__ __forceinline__ uint64_t* cell_ptr(const ExecArgs& a, uint32_t row, u
   return a.cells + (uint64_t)row * a.cols + (meta >> M_COL_SHIFT);
 }
 
+// The data of one op.  Column mode: the op's cell.  Whole-row mode
+// (kRowOps): all cells of the row, loaded together and folded into the row
+// digest, or all stored with write_cell values.
+__device__ __forceinline__ uint64_t row_digest(const uint64_t* r, uint32_t cols) {
+  uint64_t v[ROW_MAX_COLS];
+#pragma unroll
+  for (int i = 0; i < ROW_MAX_COLS; ++i) v[i] = (uint32_t)i < cols ? ld(r + i) : 0;
+  uint64_t h = DIGEST_INIT;
+#pragma unroll
+  for (int i = 0; i < ROW_MAX_COLS; ++i)
+    if ((uint32_t)i < cols) h = digest_step(h, v[i]);
+  return h;
+}
+__device__ __forceinline__ uint64_t op_load(const ExecArgs& a, uint32_t row, uint32_t meta) {
+  if constexpr (!kRowOps)
+    return ld(cell_ptr(a, row, meta));
+  else
+    return row_digest(a.cells + (uint64_t)row * a.cols, a.cols);
+}
+__device__ __forceinline__ void op_store(const ExecArgs& a, uint32_t row, uint32_t meta, uint64_t tx, uint32_t j) {
+  if constexpr (!kRowOps) {
+    st(cell_ptr(a, row, meta), write_value(a.seed, tx, j));
+  } else {
+    uint64_t* r = a.cells + (uint64_t)row * a.cols;
+#pragma unroll
+    for (int i = 0; i < ROW_MAX_COLS; ++i)
+      if ((uint32_t)i < a.cols) st(r + i, write_cell(a.seed, tx, j, (uint32_t)i));
+  }
+}
+
 template <int G>
 __device__ __forceinline__ uint32_t op_index(int k) {
   return (lane_id() & (G - 1)) + (uint32_t)k * G;
+}
+
+// Event log (kEvents): an op's event is stamped when its access takes effect,
+// a transaction's commit event after all of its ops' events (the group's
+// first lane stamps it after the group's ops were stamped in program order).
+// Stamps come from one global counter, like AddEvent's (runtime.cuh:64-81);
+// a retry overwrites the slots of the attempt before, so the log holds the
+// committed attempt.
+template <int G>
+__device__ __forceinline__ void ev_op(const TxCtx<G>& c, const ExecArgs& a, int k) {
+  if constexpr (kEvents) a.ev_op[c.op0 + op_index<G>(k)] = fetch_add(a.ctrs + CTR_EVENT, 1) + 1;
+}
+template <int G>
+__device__ __forceinline__ void ev_commit(const TxCtx<G>& c, const ExecArgs& a) {
+  if constexpr (kEvents) {
+    if ((lane_id() & (G - 1)) == 0) a.ev_commit[c.tx] = fetch_add(a.ctrs + CTR_EVENT, 1) + 1;
+  }
 }
 
 // Abort bookkeeping, identical on every lane of the group (the wall clock is a
@@ -287,11 +361,10 @@ struct TwoPL {
         const uint32_t m = c.meta[k];
         v[k] = 0;
         if (!(m & M_VALID)) continue;
-        uint64_t* cell = cell_ptr(a, c.row[k], m);
         if (m & M_WRITE)
-          st(cell, write_value(a.seed, c.tx, op_index<G>(k)));
+          op_store(a, c.row[k], m, c.tx, op_index<G>(k));
         else
-          v[k] = ld(cell);
+          v[k] = op_load(a, c.row[k], m);
       }
     }
     uint64_t ticket = wave_fetch_inc(a.ctrs + CTR_TICKET, commit && (lane_id() & (G - 1)) == 0);
@@ -301,6 +374,13 @@ struct TwoPL {
       for (int k = 0; k < K; ++k) {
         const uint32_t m = c.meta[k];
         if ((m & M_VALID) && !(m & M_WRITE)) a.reads[c.op0 + op_index<G>(k)] = v[k];
+      }
+      if constexpr (kEvents) {
+        // every lock is held: the accesses take effect now, the commit before the unlocks
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (c.meta[k] & M_VALID) ev_op<G>(c, a, k);
+        ev_commit<G>(c, a);
       }
       order_release();
 #pragma unroll
@@ -373,7 +453,7 @@ __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs&
   }
   order_acquire();
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = cand[k] ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
+  for (int k = 0; k < K; ++k) v[k] = cand[k] ? op_load(a, c.row[k], c.meta[k]) : 0;
   order_acquire();
   uint64_t ver[K];
   bool ok[K];
@@ -398,22 +478,6 @@ __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs&
 struct TimestampOrdering {
   static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
   static constexpr bool kBulkRefill = true;
-  static __device__ StepResult read(uint64_t* w, uint64_t* cell, uint64_t ts, uint64_t& val, uint64_t& seen) {
-    uint64_t cur = ld(w);
-#pragma unroll 1
-    for (int t = 0; t < CAS_TRIES; ++t) {
-      if (tsw::wts(cur) > ts) {
-        seen = tsw::wts(cur);
-        return R_ABORT;
-      }
-      if (tsw::unc(cur)) return R_WAIT;
-      order_acquire();
-      val = ld(cell);
-      order_acquire();
-      if (hold_version(w, ts, tsw::version(cur), cur, false)) return R_OK;
-    }
-    return R_WAIT;
-  }
   static __device__ StepResult write_mark(uint64_t* w, uint64_t ts, uint64_t& prev, uint64_t& seen) {
     uint64_t cur = ld(w);
 #pragma unroll 1
@@ -462,6 +526,7 @@ struct TimestampOrdering {
         if (r[k] == R_OK) {
           c.meta[k] |= M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v[k];
+          ev_op<G>(c, a, k);
         } else if (r[k] == R_ABORT) {
           // a younger write committed (to.cuh:148-152)
           const uint64_t y = tsw::wts(cur[k]);
@@ -485,9 +550,10 @@ struct TimestampOrdering {
         const uint32_t m = c.meta[k];
         if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_DONE)) continue;
         const StepResult r = write_mark(a.cc + c.row[k], c.ts, c.st[k], seen);
-        if (r == R_OK)
+        if (r == R_OK) {
           c.meta[k] = m | M_DONE | M_HELD;
-        else if (r == R_ABORT)
+          ev_op<G>(c, a, k);
+        } else if (r == R_ABORT)
           abort = true;
         else
           pending = true;
@@ -507,8 +573,8 @@ struct TimestampOrdering {
     // commit (to.cuh:117-134)
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD)
-        st(cell_ptr(a, c.row[k], c.meta[k]), write_value(a.seed, c.tx, op_index<G>(k)));
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
+    ev_commit<G>(c, a);
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -526,23 +592,27 @@ struct TimestampOrdering {
 // aborts; it waits while the head is uncommitted (mvcc.cuh:169-203).  Writes
 // abort like T/O (mvcc.cuh:240-241).  Reads run before write marks so that an
 // attempt never waits while it holds an uncommitted mark.
-// Node layout (3 x u64): [0] ts word of the superseded version,
-// [1] old value of the written column, [2] previous node + 1 | column << 32.
+// Node layout (mvcc_stride u64): [0] ts word of the superseded version,
+// [1] previous node + 1 | column << 32, [2..] the superseded data: the old
+// value of the written column, or in whole-row mode the old row (cols cells).
 // ===========================================================================
+constexpr uint64_t MVCC_COL_STRIDE = 3;
+__device__ __forceinline__ uint64_t mvcc_stride(const ExecArgs& a) { return kRowOps ? 2 + (uint64_t)a.cols : MVCC_COL_STRIDE; }
 struct Mvcc {
   static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
   static constexpr bool kBulkRefill = true;
   static __device__ StepResult read(const ExecArgs& a, uint32_t row, uint32_t col, uint64_t ts,
                                     uint64_t& val) {
     uint64_t* w = a.cc + row;
-    uint64_t* cell = a.cells + (uint64_t)row * a.cols + col;
+    const uint32_t meta = col << M_COL_SHIFT;
+    const uint64_t stride = mvcc_stride(a);
     uint64_t cur = ld(w);
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
       if (tsw::unc(cur)) return R_WAIT;
       if (tsw::wts(cur) <= ts) {
         order_acquire();
-        val = ld(cell);
+        val = op_load(a, row, meta);
         order_acquire();
         if (hold_version(w, ts, tsw::version(cur), cur, false)) return R_OK;
         continue;
@@ -555,7 +625,7 @@ struct Mvcc {
       // reader far behind a hot row's head still finishes.
       order_acquire();
       uint32_t p = ld32(a.vprev + row);
-      uint64_t v = ld(cell);
+      uint64_t v = op_load(a, row, meta);
       order_acquire();
       const uint64_t after = ld(w);
       if (tsw::version(after) != tsw::version(cur)) {
@@ -564,6 +634,7 @@ struct Mvcc {
       }
       uint64_t cur_wts = tsw::wts(cur);
       bool broken = false;
+      uint32_t last = 0;  // whole-row mode: the last node walked holds the row read
 #pragma unroll 1
       while (cur_wts > ts) {
         // a link read from shared memory is an index: never follow one that
@@ -572,9 +643,10 @@ struct Mvcc {
           broken = true;
           break;
         }
-        const uint64_t* node = a.pool + (uint64_t)(p - 1) * 3;
-        const uint64_t link = ld(node + 2);
-        if ((uint32_t)(link >> 32) == col) v = ld(node + 1);
+        const uint64_t* node = a.pool + (uint64_t)(p - 1) * stride;
+        const uint64_t link = ld(node + 1);
+        if (!kRowOps && (uint32_t)(link >> 32) == col) v = ld(node + 2);
+        last = p;
         cur_wts = tsw::wts(ld(node + 0));
         p = (uint32_t)link;
       }
@@ -583,6 +655,7 @@ struct Mvcc {
         val = 0;
         return R_OK;
       }
+      if (kRowOps && last) v = row_digest(a.pool + (uint64_t)(last - 1) * stride + 2, a.cols);
       val = v;
       return R_OK;
     }
@@ -624,6 +697,7 @@ struct Mvcc {
         if (rk == R_OK) {
           c.meta[k] = m | M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v;
+          ev_op<G>(c, a, k);
         } else {
           pending = true;
         }
@@ -657,9 +731,10 @@ struct Mvcc {
             break;
           }
         }
-        if (r == R_OK)
+        if (r == R_OK) {
           c.meta[k] = m | M_DONE | M_HELD;
-        else if (r == R_ABORT)
+          ev_op<G>(c, a, k);
+        } else if (r == R_ABORT)
           abort = true;
         else
           pending = true;
@@ -690,12 +765,13 @@ struct Mvcc {
     // version nodes and new cells; then, after a release, the chain heads;
     // then, after another release, the committed words.  A reader that sees
     // a committed word therefore sees its chain head and every node below it.
+    const uint64_t stride = mvcc_stride(a);
     uint64_t oldv[K];
     uint32_t prev[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const bool h = (c.meta[k] & M_HELD) != 0;
-      oldv[k] = h ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
+      oldv[k] = h && !kRowOps ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
       prev[k] = h ? ld32(a.vprev + c.row[k]) : 0;
     }
     uint64_t nd_of[K];
@@ -705,19 +781,31 @@ struct Mvcc {
       nd_of[k] = ~0ull;
       if (!(m & M_HELD)) continue;
       if (node < a.pool_cap) {
-        uint64_t* nd = a.pool + node * 3;
+        uint64_t* nd = a.pool + node * stride;
         st(nd + 0, c.st[k]);
-        st(nd + 1, oldv[k]);
-        st(nd + 2, (uint64_t)prev[k] | ((uint64_t)(m >> M_COL_SHIFT) << 32));
+        st(nd + 1, (uint64_t)prev[k] | ((uint64_t)(m >> M_COL_SHIFT) << 32));
+        if constexpr (!kRowOps) {
+          st(nd + 2, oldv[k]);
+        } else {
+          // the whole superseded row (this attempt holds the row's mark)
+          const uint64_t* r = a.cells + (uint64_t)c.row[k] * a.cols;
+#pragma unroll
+          for (int i = 0; i < ROW_MAX_COLS; ++i)
+            if ((uint32_t)i < a.cols) st(nd + 2 + i, ld(r + i));
+        }
         nd_of[k] = node;
       }
-      st(cell_ptr(a, c.row[k], m), write_value(a.seed, c.tx, op_index<G>(k)));
       node++;
     }
+    if constexpr (kRowOps) order_release();  // the old row is saved before it is overwritten
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
       if (nd_of[k] != ~0ull) st32(a.vprev + c.row[k], (uint32_t)(nd_of[k] + 1));
+    ev_commit<G>(c, a);
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -747,7 +835,7 @@ __device__ __forceinline__ void occ_snapshot(const TxCtx<G>& c, const ExecArgs& 
   }
   order_acquire();
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = rd[k] ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
+  for (int k = 0; k < K; ++k) v[k] = rd[k] ? op_load(a, c.row[k], c.meta[k]) : 0;
   order_acquire();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -785,6 +873,30 @@ __device__ __forceinline__ void occ_lock(const TxCtx<G>& c, const ExecArgs& a, u
 struct Silo {
   static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
   static constexpr bool kBulkRefill = true;
+  // Read: the word, then the cell (silo.cuh:223-259 loads the word a second
+  // time after the data).  The commit validation re-reads every read word
+  // and aborts unless it still holds the TID seen here, unlocked; a writer
+  // that changed the cell in between must have locked the word and installed
+  // a new TID, so a torn read never commits, and the second load (one more
+  // line fetched per read) is left out.
+  template <int G>
+  static __device__ __forceinline__ void snapshot(const TxCtx<G>& c, const ExecArgs& a, uint64_t (&w1)[TxCtx<G>::K],
+                                                  uint64_t (&v)[TxCtx<G>::K], bool (&rd)[TxCtx<G>::K],
+                                                  bool (&ok)[TxCtx<G>::K]) {
+    constexpr int K = TxCtx<G>::K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t m = c.meta[k];
+      rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
+      w1[k] = rd[k] ? ld(a.cc + c.row[k]) : 0;
+    }
+    order_acquire();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      ok[k] = rd[k] && !silo::locked(w1[k]);
+      v[k] = ok[k] ? op_load(a, c.row[k], c.meta[k]) : 0;
+    }
+  }
   template <int G>
   static __device__ void round(TxCtx<G>& c, const ExecArgs& a) {
     constexpr int K = TxCtx<G>::K;
@@ -799,7 +911,7 @@ struct Silo {
         bool pending = false;
         uint64_t w1[K], v[K];
         bool rd[K], ok[K];
-        occ_snapshot<G>(c, a, w1, v, rd, ok);
+        snapshot<G>(c, a, w1, v, rd, ok);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (!rd[k]) continue;
@@ -810,6 +922,7 @@ struct Silo {
           c.st[k] = silo::ts(w1[k]);
           c.meta[k] |= M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v[k];
+          ev_op<G>(c, a, k);
         }
         if (group_all<G>(!pending))
           c.ph = PH_LOCK;
@@ -830,6 +943,7 @@ struct Silo {
           if (got[k]) {
             c.st[k] = silo::ts(old[k]);
             c.meta[k] |= M_HELD;
+            ev_op<G>(c, a, k);
           } else {
             conflict = true;
           }
@@ -870,10 +984,10 @@ struct Silo {
       return;
     }
     // write phase (silo.cuh:323-338)
+    ev_commit<G>(c, a);
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD)
-        st(cell_ptr(a, c.row[k], c.meta[k]), write_value(a.seed, c.tx, op_index<G>(k)));
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -921,6 +1035,7 @@ struct Tictoc {
           c.st[k] = w1[k];
           c.meta[k] |= M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v[k];
+          ev_op<G>(c, a, k);
         }
         if (group_all<G>(!pending))
           c.ph = PH_LOCK;
@@ -941,6 +1056,7 @@ struct Tictoc {
           if (got[k]) {
             c.st[k] = old[k];
             c.meta[k] |= M_HELD;
+            ev_op<G>(c, a, k);
           } else {
             conflict = true;
           }
@@ -969,7 +1085,9 @@ struct Tictoc {
             if (!(m & M_VALID) || (m & M_WRITE)) continue;
             if (tictoc::rts(c.st[k]) >= commit_ts) continue;
             uint64_t* w = a.cc + c.row[k];
-            uint64_t cur = ld(w);
+            // first CAS from the snapshot word: if the word is unchanged it
+            // succeeds without a load; otherwise it returns the current word
+            uint64_t cur = c.st[k];
 #pragma unroll 1
             while (true) {
               const uint64_t wts = tictoc::wts(cur);
@@ -1001,10 +1119,10 @@ struct Tictoc {
     if (!commit) return;
     ticket = group_bcast64<G>(ticket);
     // write phase (tictoc.cuh:352-368)
+    ev_commit<G>(c, a);
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD)
-        st(cell_ptr(a, c.row[k], c.meta[k]), write_value(a.seed, c.tx, op_index<G>(k)));
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -1067,11 +1185,11 @@ struct Ordered {
         if (!ready[k]) continue;
         const uint32_t m = c.meta[k];
         const uint32_t j = op_index<G>(k);
-        uint64_t* cell = cell_ptr(a, c.row[k], m);
         if (m & M_WRITE)
-          st(cell, write_value(a.seed, c.tx, j));
+          op_store(a, c.row[k], m, c.tx, j);
         else
-          a.reads[c.op0 + j] = ld(cell);
+          a.reads[c.op0 + j] = op_load(a, c.row[k], m);
+        ev_op<G>(c, a, k);
       }
       order_release();
 #pragma unroll
@@ -1083,6 +1201,7 @@ struct Ordered {
     }
     if (group_all<G>(!pending)) {
       const uint64_t key0 = a.tx_rank ? a.tx_rank[c.tx] : c.tx;
+      ev_commit<G>(c, a);
       finish_tx(c, a, now, key0, c.tx);
     } else {
       c.m_wait += now - c.t_round;
@@ -1112,9 +1231,16 @@ struct Ordered {
 // Persistent form (PERSIST = true): one launch over the whole stream; groups
 // take the next transaction as soon as theirs commits (run_persistent).
 // ===========================================================================
+// Inputs of a launch (offsets, ops, rows, turns): plain loads, or coherent
+// loads in the open form, where the host writes them while the launch runs
+// and a line cached earlier by one XCD's L2 could be stale.
+__device__ __forceinline__ uint64_t ld_in(const uint64_t* p, bool coherent) { return coherent ? ld(p) : *p; }
+__device__ __forceinline__ uint32_t ld_in32(const uint32_t* p, bool coherent) { return coherent ? ld32(p) : *p; }
+
 template <int G>
 __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t slot, uint64_t now) {
   constexpr int K = TxCtx<G>::K;
+  const bool coh = a.open != 0;
   c.live = slot < a.slot_hi;
   c.done = !c.live;
   c.ph = PH_START;
@@ -1123,9 +1249,9 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
   c.t_start = now;
   c.nops = 0;
   if (c.live) {
-    c.tx = a.tx_map ? a.tx_map[slot] : slot;
-    c.op0 = a.tx_off[c.tx];
-    c.nops = (uint32_t)(a.tx_off[c.tx + 1] - c.op0);
+    c.tx = a.tx_map ? ld_in32(a.tx_map + slot, coh) : slot;
+    c.op0 = ld_in(a.tx_off + c.tx, coh);
+    c.nops = (uint32_t)(ld_in(a.tx_off + c.tx + 1, coh) - c.op0);
   }
   // Every slot's op, then every slot's row, each step issued for all slots
   // before its wait: rows from the probe pass, or from the hash index (all
@@ -1137,11 +1263,11 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     has[k] = c.live && op_index<G>(k) < c.nops;
-    opv[k] = has[k] ? a.ops[c.op0 + op_index<G>(k)] : 0ull;
+    opv[k] = has[k] ? ld_in(a.ops + c.op0 + op_index<G>(k), coh) : 0ull;
   }
   if (a.rows_in) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) rowv[k] = has[k] ? a.rows_in[c.op0 + op_index<G>(k)] : IDX_MISS;
+    for (int k = 0; k < K; ++k) rowv[k] = has[k] ? ld_in32(a.rows_in + c.op0 + op_index<G>(k), coh) : IDX_MISS;
   } else {
     uint4 q0[K], q1[K];
     const uint32_t* bk[K];
@@ -1191,7 +1317,7 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
       if (row < a.rows) {
         c.row[k] = row;
         c.meta[k] = M_VALID | (op_is_write(op) ? M_WRITE : 0u) | (op_col(op) << M_COL_SHIFT);
-        if (a.need) c.st[k] = a.need[c.op0 + j];  // ordered schemes: the op's turn on its row
+        if (a.need) c.st[k] = ld_in32(a.need + c.op0 + j, coh);  // ordered schemes: the op's turn on its row
       } else {
         c.m_miss++;
       }
@@ -1225,84 +1351,55 @@ __device__ void watchdog_stop(const TxCtx<G>& c, const ExecArgs& a) {
   }
 }
 
-// Sum over the wave (a block can end in a half wave: lanes 0..31 only).
-__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
-  const bool full_wave = (ballot(true) >> 32) != 0;
-#pragma unroll
-  for (int m = 1; m < 32; m <<= 1) x += shfl_xor64(x, m);
-  if (full_wave) x += shfl_xor64(x, 32);
-  return x;
-}
-
 // ---------------------------------------------------------------------------
 // Persistent form.  Groups take stream slots, in stream order, from one
 // wave-aggregated counter as soon as their transaction commits, so a group
 // held up by a slow transaction never holds back later slots, and the lowest
 // unfinished slot always belongs to a running group (which the ordered
-// schemes need: they wait on lower slots only).
+// schemes need: they wait on lower slots only).  The grid holds the
+// admission window: at most that many groups, so at most that many
+// transactions in flight.
 //
-// Admission window W: a group whose rank is at or above W takes no new slot
-// (it parks).  Static window (adapt = 0): the grid itself holds W groups (or
-// as many as the device keeps resident), nobody parks.  Adaptive window
-// (adapt = 1): the grid fills the device and W moves with contention: waves
-// add their commits and aborted attempts to sharded counters every
-// FLUSH_TICKS; every CTL_TICKS one of a few controller waves compares the
-// aborted attempts per commit since the last update with two bounds: above 2,
-// W shrinks by a quarter (not below WIN_MIN); below 1/2, W grows by half plus
-// 64 (up to the grid's groups).  Group ranks interleave the blocks, so any
-// window spreads over the whole device.
+// Open form (a.open, the serving loop): the launch starts before its
+// transactions have arrived.  The host admits them batch by batch while the
+// launch runs, by raising the admission word (CTR_ADMIT0, one copy per gate
+// shard) after their ops are in device memory, and closes the launch by
+// setting the word's top bit once nothing more will come.  A group draws its
+// next slot as in the closed form and holds it until the slot is admitted
+// (then it runs it) or the launch is closed below it (then it is finished).
+// Ops, offsets, rows and turns written while the launch runs are read with
+// coherent loads.
 //
 // Exits: a group is finished when the counter has passed the end of the
-// stream; a wave leaves when all its groups are finished, when a live
-// transaction outlives the watchdog, or when it is parked and the slot
-// counter has not moved for the watchdog period.
+// stream (or of a closed launch); a wave leaves when all its groups are
+// finished, when a live transaction outlives the watchdog, when it is held
+// back (urgent state) and the slot counter has not moved for the watchdog
+// period, or when it waits for an admission that has not moved for the
+// watchdog period.
 // ---------------------------------------------------------------------------
-constexpr uint64_t FLUSH_TICKS = 10000;  // 100 us of the 100 MHz wall clock
-constexpr uint64_t CTL_TICKS = 20000;    // 200 us
 constexpr uint64_t GATE_TICKS = 5000;    // a refilling wave re-reads its gate copy at most every 50 us
-constexpr uint64_t PARK_TICKS = 20000;   // a wave with only parked groups: every 200 us
-constexpr uint64_t STALL_TICKS = 100000; // parked waves sample the slot counter every 1 ms
-constexpr uint64_t WIN_MIN = 256;
-constexpr uint32_t CTL_BLOCK_STRIDE = 64;  // wave 0 of every 64th block may run the controller
-
-__device__ __forceinline__ void adapt_window(const ExecArgs& a, uint64_t now, uint64_t wmax) {
-  uint64_t* ctl = a.ctrs + CTR_CTL;
-  uint64_t due = ld(ctl);
-  if (now < due || !cas(ctl, due, now + CTL_TICKS)) return;
-  uint64_t cm = 0, ab = 0;
-#pragma unroll 1
-  for (int s = 0; s < STAT_SHARDS; ++s) {
-    cm += ld(a.ctrs + CTR_STAT0 + 16 * s);
-    ab += ld(a.ctrs + CTR_STAT0 + 16 * s + 1);
-  }
-  const uint64_t dc = cm - ld(ctl + 1), da = ab - ld(ctl + 2);
-  if (dc + da < 256) return;  // too few events to decide on
-  st(ctl + 1, cm);
-  st(ctl + 2, ab);
-  uint64_t w = ld(a.ctrs + CTR_GATE0);
-  if (w > wmax) w = wmax;
-  if (da > 2 * dc)
-    w = w - w / 4 < WIN_MIN ? WIN_MIN : w - w / 4;
-  else if (2 * da < dc)
-    w = w + w / 2 + 64 > wmax ? wmax : w + w / 2 + 64;
-#pragma unroll 1
-  for (int s = 0; s < GATE_SHARDS; ++s) st(a.ctrs + CTR_GATE0 + 16 * s, w);
-}
+constexpr uint64_t PARK_TICKS = 20000;   // a wave with no live group: every 200 us
+constexpr uint64_t ADMIT_TICKS = 1000;   // a wave whose groups wait for an admission: every 10 us
+constexpr uint64_t STALL_TICKS = 100000; // held-back waves sample the slot counter every 1 ms
+constexpr uint64_t NO_SLOT = ~0ull;
 
 template <class S, int G>
-__device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups) {
+__device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a) {
   const bool leader = (lane_id() & (G - 1)) == 0;
-  const uint64_t grank = (uint64_t)(threadIdx.x / G) * gridDim.x + blockIdx.x;
   const uint64_t nslots = a.slot_hi - a.slot_lo;
-  const bool adapt = a.adapt != 0;
-  const bool controller = adapt && threadIdx.x < 64 && (blockIdx.x % CTL_BLOCK_STRIDE) == 0;
+  const bool open = a.open != 0;
   uint64_t* slot_ctr = a.ctrs + CTR_SLOT;
-  uint64_t* gate = a.ctrs + CTR_GATE0 + 16 * (blockIdx.x & (GATE_SHARDS - 1));
+  const uint32_t shard = blockIdx.x & (GATE_SHARDS - 1);
+  const uint64_t* gate = a.ctrs + CTR_GATE0 + 16 * shard;
+  const uint32_t* admit = reinterpret_cast<const uint32_t*>(a.ctrs + CTR_ADMIT0 + 16 * shard);
   bool finished = false;
-  uint64_t acc_c = 0, acc_a = 0;
+  uint64_t held = NO_SLOT;  // slot drawn and not yet running (open form: waiting for its admission)
   uint64_t now = now_ticks();
-  uint64_t next_flush = now + FLUSH_TICKS, next_ctl = now, next_gate = now, next_stall = now;
-  uint64_t wcache = ~0ull, urgent = 0, handed_out = 0;
+  uint64_t next_gate = now, next_stall = now;
+  uint64_t urgent = 0, handed_out = 0;
+  uint64_t admitted = open ? 0 : nslots;
+  bool closed = !open;
+  uint64_t adm_seen = 0, adm_seen_at = now;
   uint64_t seen_slot = ~0ull, seen_at = now;
   c.live = false;
   c.done = true;
@@ -1323,35 +1420,59 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups)
     const uint32_t nneed = (uint32_t)__popcll(ballot(need && leader));
     const bool bulk = !S::kBulkRefill || 2 * nneed >= ngw || ballot(!c.done) == 0;
     if (bulk && ballot(need) != 0) {
-      if (now >= next_gate) {
-        // this block's gate copy: window, urgent count, stream handed out
-        const uint64_t gw = ld(gate), gu = ld(gate + 1), gd = ld(gate + 2);
-        wcache = adapt ? gw : ~0ull;
-        urgent = gu;
-        handed_out = gd;
-        next_gate = now + (ballot(!c.done) != 0 ? GATE_TICKS : PARK_TICKS);
+      const bool read_gate = now >= next_gate;
+      if (read_gate) {
+        // this block's gate copy: urgent count, stream handed out; and, in
+        // the open form, its copy of the admission word
+        urgent = ld(gate + 1);
+        handed_out = ld(gate + 2);
+        if (open) {
+          const uint32_t w = ld32(admit);
+          admitted = w & ~ADMIT_CLOSED;
+          admitted = admitted < nslots ? admitted : nslots;
+          closed = (w & ADMIT_CLOSED) != 0;
+          if (admitted != adm_seen) {
+            adm_seen = admitted;
+            adm_seen_at = now;
+          }
+        }
       }
-      if (handed_out && need) finished = true;
-      const bool take = need && !finished && urgent == 0 && grank < wcache;
+      if (handed_out && need && held == NO_SLOT) finished = true;
+      const bool take = need && !finished && urgent == 0 && held == NO_SLOT;
       if (ballot(take) != 0) {
         const uint64_t got = group_bcast64<G>(wave_fetch_inc(slot_ctr, take && leader));
         if (take) {
           if (got < nslots) {
-            load_tx(c, a, a.slot_lo + got, now_ticks());
+            held = got;
           } else {
             finished = true;
             // the group that drew the first slot past the end tells every gate
             if (got == nslots && leader) {
 #pragma unroll 1
-              for (int s = 0; s < GATE_SHARDS; ++s) st(a.ctrs + CTR_GATE0 + 16 * s + 2, 1);
+              for (int g = 0; g < GATE_SHARDS; ++g) st(a.ctrs + CTR_GATE0 + 16 * g + 2, 1);
             }
           }
         }
       }
-      const bool parked = need && !finished && !take;
+      if (need && held != NO_SLOT) {
+        if (held < admitted) {
+          if (open) order_acquire();  // the admission word before the data it admits
+          load_tx(c, a, a.slot_lo + held, now_ticks());
+          held = NO_SLOT;
+        } else if (closed) {
+          finished = true;  // the launch closed below this slot
+        }
+      }
+      if (read_gate) {
+        // groups that wait for an admission poll their copy often; refilling
+        // waves now and then; waves with no live group rarely
+        const bool waiting = ballot(!finished && held != NO_SLOT) != 0;
+        next_gate = now + (open && waiting ? ADMIT_TICKS : ballot(!c.done) != 0 ? GATE_TICKS : PARK_TICKS);
+      }
+      // groups held back by the urgent state: a slot counter that has not
+      // moved for the watchdog period means that no group progresses
+      const bool parked = need && !finished && held == NO_SLOT && !take;
       if (ballot(parked) != 0 && now >= next_stall) {
-        // parked groups: a slot counter that has not moved for the watchdog
-        // period means that no group progresses
         next_stall = now + STALL_TICKS;
         const uint64_t cur = ld(slot_ctr);
         if (cur >= nslots) {
@@ -1363,41 +1484,23 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a, uint64_t ngroups)
           stalled = true;
         }
       }
+      // open form: an admission that has not moved for the watchdog period
+      // (the host neither admitted nor closed) ends the wait
+      if (open && need && !finished && held != NO_SLOT && a.watchdog_ticks && now - adm_seen_at > a.watchdog_ticks)
+        stalled = true;
     }
     if (ballot(!finished) == 0) break;
     if (ballot(stalled) != 0) {
       watchdog_stop(c, a);
       break;
     }
-    const uint32_t att0 = c.attempts;
-    const bool done0 = c.done;
     if (ballot(!c.done) == 0)
-      __builtin_amdgcn_s_sleep(64);  // every group of the wave is parked
+      __builtin_amdgcn_s_sleep(64);  // no group of the wave is live (held back, waiting or finished)
     else if (ballot(!c.done && !((c.ph == PH_START || S::kRestInOps) && c.t_round < c.t_resume)) == 0)
       __builtin_amdgcn_s_sleep(4);  // every live group is resting
     else
       S::round(c, a);
     c.t_round = now = now_ticks();
-    if (adapt) {
-      if (leader) {
-        acc_a += c.attempts - att0;
-        acc_c += (!done0 && c.done) ? 1u : 0u;
-      }
-      if (now >= next_flush) {
-        next_flush = now + FLUSH_TICKS;
-        const uint64_t sc = wave_sum(acc_c), sa = wave_sum(acc_a);
-        acc_c = acc_a = 0;
-        if (lane_id() == 0 && (sc | sa)) {
-          uint64_t* stat = a.ctrs + CTR_STAT0 + 16 * (blockIdx.x & (STAT_SHARDS - 1));
-          if (sc) fetch_add(stat, sc);
-          if (sa) fetch_add(stat + 1, sa);
-        }
-      }
-      if (controller && now >= next_ctl) {
-        next_ctl = now + CTL_TICKS / 2;
-        if (lane_id() == 0) adapt_window(a, now, ngroups);
-      }
-    }
     const bool stuck = !c.done && a.watchdog_ticks && now - c.t_start > a.watchdog_ticks;
     if (ballot(stuck) != 0) {
       watchdog_stop(c, a);
@@ -1414,7 +1517,7 @@ template <class S, int G, bool PERSIST, int MAXT = 1024>
 __global__ __launch_bounds__(MAXT) void txn_exec(ExecArgs a) {
   TxCtx<G> c;
   if constexpr (PERSIST) {
-    run_persistent<S, G>(c, a, ((uint64_t)gridDim.x * blockDim.x) / G);
+    run_persistent<S, G>(c, a);
   } else {
     // plain form: one transaction per group
     const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1473,5 +1576,5 @@ __global__ __launch_bounds__(MAXT) void txn_exec(ExecArgs a) {
   }
 }
 
-}  // namespace dev
+}  // namespace dev (dev_row, dev_evt, dev_row_evt)
 }  // namespace gcctb

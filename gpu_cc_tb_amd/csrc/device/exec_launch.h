@@ -6,7 +6,15 @@
 #include "launch.h"
 
 namespace gcctb {
+#if GCCTB_ROW_OPS && GCCTB_EVENTS
+namespace dev_row_evt {
+#elif GCCTB_ROW_OPS
+namespace dev_row {
+#elif GCCTB_EVENTS
+namespace dev_evt {
+#else
 namespace dev {
+#endif
 
 template <class S, int G, bool P, int MAXT>
 hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s) {
@@ -23,11 +31,14 @@ hipError_t launch_one(const ExecArgs& a, uint64_t nslots, int threads, hipStream
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&txn_exec<S, G, P, MAXT>),
                                                        threads, 0);
     if (e != hipSuccess) return e;
-    const uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 1) * (uint64_t)(cus > 0 ? cus : 1);
+    uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 1) * (uint64_t)(cus > 0 ? cus : 1);
+    // The open form runs beside the admission copies and preparation kernels
+    // that feed it, on another stream: it leaves half of the block slots free
+    // for them (a full device would hold those kernels back until it ends).
+    if (a.open && resident > 1) resident /= 2;
     if (blocks > resident) blocks = resident;
-    // static admission window: fewer transactions in flight under heavy
-    // contention (the adaptive form keeps the whole grid and parks groups)
-    if (a.persist_groups && !a.adapt) {
+    // admission window: fewer transactions in flight under heavy contention
+    if (a.persist_groups) {
       const uint64_t wblocks = (a.persist_groups * G + threads - 1) / threads;
       if (wblocks < blocks) blocks = wblocks < 1 ? 1 : wblocks;
     }
@@ -44,12 +55,17 @@ hipError_t launch_exec_g(int G, const ExecArgs& args, uint64_t nslots, int threa
   ExecArgs a = args;
   a.persist = persist ? 1u : 0u;
   // persistent launches of at most 256 threads per block take the
-  // instantiation bounded by that block size (more registers, no spills)
-#define GC_CASE(g)                                                                  \
-  case g:                                                                           \
-    return persist ? (threads <= 256 ? launch_one<S, g, true, 256>(a, nslots, threads, s)    \
-                                     : launch_one<S, g, true, 1024>(a, nslots, threads, s))  \
-                   : launch_one<S, g, false, 1024>(a, nslots, threads, s);
+  // instantiation bounded by that block size (more registers, no spills);
+  // whole-row mode and the event log have the plain (per-batch) kernels only
+#define GC_CASE(g)                                                                      \
+  case g:                                                                               \
+    if constexpr (kRowOps || kEvents) {                                                 \
+      return persist ? hipErrorInvalidValue : launch_one<S, g, false, 1024>(a, nslots, threads, s); \
+    } else {                                                                            \
+      return persist ? (threads <= 256 ? launch_one<S, g, true, 256>(a, nslots, threads, s)    \
+                                       : launch_one<S, g, true, 1024>(a, nslots, threads, s))  \
+                     : launch_one<S, g, false, 1024>(a, nslots, threads, s);            \
+    }
   switch (G) {
     GC_CASE(1)
     GC_CASE(2)
@@ -62,5 +78,5 @@ hipError_t launch_exec_g(int G, const ExecArgs& args, uint64_t nslots, int threa
 #undef GC_CASE
 }
 
-}  // namespace dev
+}  // namespace dev (dev_row, dev_evt, dev_row_evt)
 }  // namespace gcctb

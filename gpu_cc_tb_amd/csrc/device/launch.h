@@ -16,6 +16,11 @@ hipError_t exec_mvcc(int G, const ExecArgs& a, uint64_t nslots, int threads, boo
 hipError_t exec_silo(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
 hipError_t exec_tictoc(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
 hipError_t exec_ordered(int G, const ExecArgs& a, uint64_t nslots, int threads, bool persist, hipStream_t s);
+// kernel variants (plain form only), any scheme: whole-row mode, the event
+// log, or both
+hipError_t exec_row(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_evt(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+hipError_t exec_row_evt(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
 
 // --- table / index ---
 hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s);
@@ -70,6 +75,13 @@ hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t
 // relative to the row's first access, for counters that start at zero.
 hipError_t access_need_rel(const uint32_t* sop, const uint32_t* gstart, const uint32_t* segstart, uint64_t n,
                            uint32_t* need, hipStream_t s);
+// Open form, one admitted batch sorted by (row, op) with group starts gstart
+// and row starts rstart: need[op] = base[row] + gstart[p] - rstart[p]; then
+// row_base_add adds the batch's accesses of each row to base[row].
+hipError_t access_need_open(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart, const uint32_t* rstart,
+                            uint64_t n, uint64_t nrows, const uint32_t* base, uint32_t* need, hipStream_t s);
+hipError_t row_base_add(const uint32_t* srow, const uint32_t* rstart, uint64_t n, uint64_t nrows, uint32_t* base,
+                        hipStream_t s);
 // GPUTx: *conflicts += adjacent same-row, same-rank accesses where either
 // writes, from the (row << rank_bits | rank) sorted keys and the group flags.
 hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
@@ -103,8 +115,10 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
                       uint32_t* vprev, hipStream_t s);
 hipError_t batch_begin(uint64_t* ctrs, hipStream_t s);
-// adaptive admission state of a persistent launch (initial window in transactions)
-hipError_t adapt_begin(uint64_t* ctrs, uint64_t window, hipStream_t s);
+// admission state of a persistent launch (slot counter, gates, admission words)
+hipError_t persist_begin(uint64_t* ctrs, hipStream_t s);
+// *out = the device wall clock (100 MHz), written by a one-block kernel
+hipError_t clock_sample(uint64_t* out, hipStream_t s);
 
 }  // namespace launch
 }  // namespace gcctb

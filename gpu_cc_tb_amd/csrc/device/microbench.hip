@@ -81,6 +81,35 @@ __global__ __launch_bounds__(256) void mb_ts_read(uint64_t* words, uint64_t n, u
   }
   if (acc == 0x5eed) atomicAdd(sink, 1ull);
 }
+// Hot words: every lane adds to one of 16 words, which share one 128-byte
+// line (spread = 0) or sit on 16 lines (spread = 1).  The ratio says whether
+// the atomics of different words in one line serialize like those of one
+// word (hot Zipf rows are neighbours in a dense per-row array).
+__global__ __launch_bounds__(256) void mb_add_hot16(uint64_t* words, uint64_t iters, int spread,
+                                                     unsigned long long* sink) {
+  const uint32_t w = (blockIdx.x * 7 + threadIdx.x) & 15u;
+  uint64_t* p = words + (spread ? (uint64_t)w * 16 : (uint64_t)w);
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < iters; ++i) acc += dev::fetch_add(p, 1);
+  if (acc == 0x5eed) atomicAdd(sink, 1ull);
+}
+// One hot word under atomics (lane 0 of every wave adds to word 0) while the
+// other lanes poll with coherent loads: words 1..15 of the same line
+// (spread = 0) or words on 15 other lines (spread = 1).  Rate: adds to word 0.
+__global__ __launch_bounds__(256) void mb_add_polled(uint64_t* words, uint64_t iters, int spread,
+                                                      unsigned long long* sink) {
+  const uint32_t l = dev::lane_id();
+  const uint32_t w = 1u + (blockIdx.x * 5 + threadIdx.x) % 15u;
+  const uint64_t* q = words + (spread ? (uint64_t)w * 16 : (uint64_t)w);
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < iters; ++i) {
+    if (l == 0)
+      acc += dev::fetch_add(words, 1);
+    else
+      acc += dev::ld(q);
+  }
+  if (acc == 0x5eed) atomicAdd(sink, 1ull);
+}
 __global__ __launch_bounds__(256) void mb_copy(const uint4* src, uint4* dst, uint64_t n16) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
@@ -94,8 +123,11 @@ namespace launch {
 // 3 random coherent load, 4 random cached load, 5 copy (n words -> n words),
 // 6 coherent load pairs in one 128-byte line, 7 coherent load pairs in two
 // lines, 8 / 9 load word + load cell + CAS word with the cell in the word's
-// line / in another line.  Returns the rate in operations/s (bytes/s for the
-// copy, read + write; pairs for 6 and 7, reads for 8 and 9).
+// line / in another line, 10 / 11 adds to 16 words in one line / on 16 lines,
+// 12 / 13 adds to one word while other lanes poll words of its line / of
+// other lines.  Returns the rate in operations/s (bytes/s for the copy, read
+// + write; pairs for 6 and 7, reads for 8 and 9, adds to the hot word for 12
+// and 13).
 hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint64_t iters, double* rate,
                       double* ms_out) {
   uint64_t* buf = nullptr;
@@ -124,6 +156,10 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
       case 7: hipLaunchKernelGGL(mb_load_pairs, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 0, sink); break;
       case 8: hipLaunchKernelGGL(mb_ts_read, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 1, sink); break;
       case 9: hipLaunchKernelGGL(mb_ts_read, dim3(blocks), dim3(256), 0, 0, buf, n_words, iters, 7ull, 0, sink); break;
+      case 10: hipLaunchKernelGGL(mb_add_hot16, dim3(blocks), dim3(256), 0, 0, buf, iters, 0, sink); break;
+      case 11: hipLaunchKernelGGL(mb_add_hot16, dim3(blocks), dim3(256), 0, 0, buf, iters, 1, sink); break;
+      case 12: hipLaunchKernelGGL(mb_add_polled, dim3(blocks), dim3(256), 0, 0, buf, iters, 0, sink); break;
+      case 13: hipLaunchKernelGGL(mb_add_polled, dim3(blocks), dim3(256), 0, 0, buf, iters, 1, sink); break;
       default: hipLaunchKernelGGL(mb_copy, dim3(2048), dim3(256), 0, 0, (const uint4*)buf, (uint4*)buf2, n_words / 2); break;
     }
   };
@@ -138,7 +174,7 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
   double ops;
   if (which == 5)
     ops = 2.0 * 8.0 * (double)n_words;
-  else if (which == 2)
+  else if (which == 2 || which == 12 || which == 13)
     ops = (double)((threads_total + 63) / 64) * (double)iters;
   else
     ops = (double)blocks * 256.0 * (double)iters;

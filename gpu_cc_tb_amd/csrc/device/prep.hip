@@ -536,6 +536,31 @@ __global__ void k_access_need_rel(const uint32_t* sop, const uint32_t* gstart, c
     need[sop[p] & OPID_MASK] = gstart[p] - segstart[p];
 }
 
+// Open form (serving), one admitted batch at a time, accesses sorted by
+// (row, op): the turn counts the accesses to the row that earlier batches of
+// the launch queued (base[row]) plus the group start within the batch.
+__global__ void k_access_need_open(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
+                                   const uint32_t* rstart, uint64_t n, uint64_t nrows, const uint32_t* base,
+                                   uint32_t* need) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = srow[p];
+    if (r >= nrows) continue;  // an index miss never executes
+    need[sop[p] & OPID_MASK] = base[r] + (gstart[p] - rstart[p]);
+  }
+}
+// base[row] += the batch's accesses to the row, written once per row (at the
+// row's last position); run after k_access_need_open.
+__global__ void k_row_base_add(const uint32_t* srow, const uint32_t* rstart, uint64_t n, uint64_t nrows,
+                               uint32_t* base) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = srow[p];
+    if (r >= nrows) continue;
+    if (p + 1 == n || srow[p + 1] != r) base[r] += (uint32_t)(p - rstart[p] + 1);
+  }
+}
+
 // GPUTx same-rank conflicts: adjacent accesses of one row with equal rank
 // where either writes.  In the (row << rank_bits | rank) sorted keys, same
 // row and rank means equal keys; flags[p] (access_group_flags) is non-zero
@@ -632,25 +657,21 @@ __global__ void k_batch_begin(uint64_t* ctrs) {
   }
 }
 
-// Adaptive admission state for one persistent launch: slot counter, window,
-// controller words and the per-shard commit / abort counts.
-__global__ void k_adapt_begin(uint64_t* ctrs, uint64_t window) {
+// Admission state of one persistent launch: slot counter, gate copies and
+// (open form) admission words.
+__global__ void k_persist_begin(uint64_t* ctrs) {
   const int i = threadIdx.x;
-  if (i == 0) {
-    dev::st(ctrs + CTR_SLOT, 0);
-    dev::st(ctrs + CTR_CTL, 0);
-    dev::st(ctrs + CTR_CTL + 1, 0);
-    dev::st(ctrs + CTR_CTL + 2, 0);
-  }
-  if (i < STAT_SHARDS) {
-    dev::st(ctrs + CTR_STAT0 + 16 * i, 0);
-    dev::st(ctrs + CTR_STAT0 + 16 * i + 1, 0);
-  }
+  if (i == 0) dev::st(ctrs + CTR_SLOT, 0);
   if (i < GATE_SHARDS) {
-    dev::st(ctrs + CTR_GATE0 + 16 * i, window);
     dev::st(ctrs + CTR_GATE0 + 16 * i + 1, 0);
     dev::st(ctrs + CTR_GATE0 + 16 * i + 2, 0);
+    dev::st(ctrs + CTR_ADMIT0 + 16 * i, 0);  // the u32 admission word and its neighbour
   }
+}
+
+// Device wall clock now (for mapping commit stamps to host time).
+__global__ void k_clock_sample(uint64_t* out) {
+  if (threadIdx.x == 0) dev::st(out, wall_clock64());
 }
 
 }  // namespace
@@ -777,8 +798,22 @@ hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, ui
   }
   GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, cc, vprev);
 }
-hipError_t adapt_begin(uint64_t* ctrs, uint64_t window, hipStream_t s) {
-  hipLaunchKernelGGL(k_adapt_begin, dim3(1), dim3(64), 0, s, ctrs, window);
+hipError_t access_need_open(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart, const uint32_t* rstart,
+                            uint64_t n, uint64_t nrows, const uint32_t* base, uint32_t* need, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  GC_LAUNCH(k_access_need_open, n, srow, sop, gstart, rstart, n, nrows, base, need);
+}
+hipError_t row_base_add(const uint32_t* srow, const uint32_t* rstart, uint64_t n, uint64_t nrows, uint32_t* base,
+                        hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  GC_LAUNCH(k_row_base_add, n, srow, rstart, n, nrows, base);
+}
+hipError_t persist_begin(uint64_t* ctrs, hipStream_t s) {
+  hipLaunchKernelGGL(k_persist_begin, dim3(1), dim3(64), 0, s, ctrs);
+  return hipGetLastError();
+}
+hipError_t clock_sample(uint64_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_clock_sample, dim3(1), dim3(64), 0, s, out);
   return hipGetLastError();
 }
 

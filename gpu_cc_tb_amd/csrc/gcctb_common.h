@@ -88,6 +88,18 @@ GC_HD uint64_t write_value(uint64_t seed, uint64_t tx, uint32_t op) {
   return hash3(seed ^ TAG_WRITE, tx, op);
 }
 
+// Whole-row mode (the reference's Item copies, benchmark/ycsb.cu:57,86-91):
+// every op reads or writes all cols cells of its row.  Cell c written by op
+// `op` of transaction `tx` holds write_cell(seed, tx, op, c); a read returns
+// the digest of the row it saw (cells folded in column order), which pins
+// every cell of the row in one 64-bit result.
+constexpr int ROW_MAX_COLS = 16;  // whole-row mode: at most 16 cells (128 B) per row
+constexpr uint64_t DIGEST_INIT = 0x524f570000000000ull;  // "ROW"
+GC_HD uint64_t write_cell(uint64_t seed, uint64_t tx, uint32_t op, uint32_t col) {
+  return mix64(write_value(seed, tx, op) + col);
+}
+GC_HD uint64_t digest_step(uint64_t h, uint64_t cell) { return mix64(h ^ cell); }
+
 // ---------------------------------------------------------------------------
 // CC word layouts (explicit masks; the reference uses LSB-first bitfields).
 // ---------------------------------------------------------------------------
@@ -168,18 +180,22 @@ constexpr int CTR_POOL = 16;
 constexpr int CTR_TS0 = 32;
 constexpr int TS_SHARDS = 16;
 constexpr int TS_SHARD_BITS = 4;
-// Admission state of the persistent form: next stream slot, controller
-// state, per-shard commit / abort counts, and GATE_SHARDS copies of the gate
-// a wave reads before taking slots (one copy per block index mod GATE_SHARDS,
-// each on its own line, so the polls of thousands of waves spread out):
-// [0] admission window, [1] transactions in urgent state, [2] stream handed out.
+// Admission state of the persistent form: the next stream slot, and
+// GATE_SHARDS copies of the gate a wave reads before taking slots (one copy
+// per block index mod GATE_SHARDS, each on its own line, so the polls of
+// thousands of waves spread out): [1] transactions in urgent state, [2]
+// stream handed out.  The open form (serving) adds GATE_SHARDS copies of the
+// admission word that the host writes while the launch runs: a u32 at the
+// start of each copy's line (a host copy writes an aligned 32-bit word in one
+// piece, a 64-bit word possibly in two); slots below its low 31 bits are
+// admitted, bit 31 closes the launch (no more admissions).
 constexpr int CTR_SLOT = CTR_TS0 + 16 * TS_SHARDS;
-constexpr int CTR_CTL = CTR_SLOT + 16;   // [0] next update tick, [1] commits, [2] aborts at the last update
-constexpr int CTR_STAT0 = CTR_CTL + 16;  // shard s: [16*s] commits, [16*s + 1] aborted attempts
-constexpr int STAT_SHARDS = 16;
-constexpr int CTR_GATE0 = CTR_STAT0 + 16 * STAT_SHARDS;
+constexpr int CTR_GATE0 = CTR_SLOT + 16;
 constexpr int GATE_SHARDS = 16;
-constexpr int CTR_WORDS = CTR_GATE0 + 16 * GATE_SHARDS;
+constexpr int CTR_ADMIT0 = CTR_GATE0 + 16 * GATE_SHARDS;
+constexpr int CTR_EVENT = CTR_ADMIT0 + 16 * GATE_SHARDS;  // event-log stamps (a whole run)
+constexpr int CTR_WORDS = CTR_EVENT + 16;
+constexpr uint32_t ADMIT_CLOSED = 1u << 31;
 // Waves add their totals into one of METRIC_SLOTS 128-byte slots (by block
 // index) so that the end-of-kernel atomics do not pile onto one address; the
 // host sums the slots.
@@ -260,12 +276,14 @@ struct ExecArgs {
   uint64_t watchdog_ticks; // a wave gives up after this many ticks (0 = never)
   uint64_t* dbg;           // optional: 4 u64 per op, written for transactions the watchdog stopped
   uint64_t persist_groups; // persistent form: at most this many transactions in flight (0 = device-filling)
-  uint32_t adapt;          // persistent form: adapt the window to the abort rate
+  uint32_t open;           // persistent form: slots are admitted while the launch runs (CTR_ADMIT0)
   uint32_t persist;        // 1 in persistent launches (slots from CTR_SLOT, gates, urgent state)
   uint32_t urgent_attempts; // persistent form: aborts that make a transaction urgent (0 = never)
   uint32_t backoff_exp;     // retry delay spans at most 2^backoff_exp units
   uint32_t poll_ticks;      // ordered schemes: expected ticks per handoff on a row (rest pacing)
   uint64_t* tcommit;       // optional, per tx: wall-clock tick of its commit (serving form)
+  uint64_t* ev_op;         // event-log kernels: per op, the stamp of its access (0 = none)
+  uint64_t* ev_commit;     // event-log kernels: per tx, the stamp of its commit (0 = none)
 };
 
 }  // namespace gcctb

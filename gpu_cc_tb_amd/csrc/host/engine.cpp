@@ -133,6 +133,16 @@ struct gcctb_ctx {
   bool rows_probed = false;  // rows_out holds every op's row (persistent form)
   gcctb::DevBuf<uint64_t> tcommit;  // serving form: per-tx commit tick
   bool record_commit = false;
+  // serving form: the admission stream (copies and preparation that feed a
+  // running open launch), GaccO's per-row access counts of the launch, and a
+  // device word for clock samples
+  hipStream_t stream2 = nullptr;
+  bool open_launch = false;
+  gcctb::DevBuf<uint32_t> rowbase;
+  gcctb::DevBuf<uint64_t> clock_word;
+  // event log (debug_events): per-op access stamps, per-tx commit stamps
+  gcctb::DevBuf<uint64_t> ev_op, ev_commit;
+  bool events_ready = false;
   std::vector<uint32_t> h_rank_counts;
   bool rank_counts_ready = false;  // h_rank_counts matches rank[]
   std::vector<uint64_t> custom_batches;  // uneven batch sizes (empty: batch_size)
@@ -175,6 +185,21 @@ class SchemeHost {
     (void)o1;
     HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream));
   }
+  // Serving form, open launch: reset the counters of a launch whose
+  // transactions are admitted while it runs (on the launch's stream).
+  virtual void PrepOpen() {
+    HIP_TRY(launch::batch_begin(ctx.ctrs.p, ctx.stream));
+    HIP_TRY(launch::persist_begin(ctx.ctrs.p, ctx.stream));
+  }
+  // Serving form, open launch: prepare the ops [o0, o1) of an admitted batch
+  // on the admission stream s, before the admission word covers them (rows
+  // of the probe pass where the scheme wants one).
+  virtual void PrepAdmit(uint64_t o0, uint64_t o1, hipStream_t s) {
+    const bool pass = ctx.probe_pass >= 0 ? ctx.probe_pass != 0 : WantsProbePass();
+    if (pass && o1 > o0) HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, o1 - o0, ctx.rows_out.p + o0, s));
+  }
+  // True when the open launch reads every op's row from rows_out (PrepAdmit wrote it).
+  virtual bool OpenRowsProbed() const { return ctx.probe_pass >= 0 ? ctx.probe_pass != 0 : WantsProbePass(); }
   // True when PrepStream / PrepLaunch leave the row of every op they cover in
   // rows_out, so the persistent and serving forms need no probe pass of their own.
   virtual bool ProbesRows() const { return false; }
@@ -238,8 +263,24 @@ class SchemeHost {
     a.poll_ticks = ctx.poll_ticks;
     a.dbg = ctx.dbg.p;
     a.persist_groups = ctx.cfg.persist_window;
-    a.adapt = ctx.cfg.persist_adaptive ? 1u : 0u;
+    a.open = ctx.open_launch ? 1u : 0u;
+    a.ev_op = ctx.cfg.debug_events ? ctx.ev_op.p : nullptr;
+    a.ev_commit = ctx.cfg.debug_events ? ctx.ev_commit.p : nullptr;
     return a;
+  }
+  // Kernel variants (whole-row mode, the event log): every scheme's plain
+  // kernels of a variant come from one launcher.
+  bool launch_rows(const ExecArgs& a, uint64_t n, bool persist) {
+    if (!ctx.cfg.row_ops && !ctx.cfg.debug_events) return false;
+    if (persist) throw HipFailure{hipErrorInvalidValue, "whole-row mode and the event log have no persistent kernels"};
+    const int sch = (int)ctx.scheme;
+    if (ctx.cfg.row_ops && ctx.cfg.debug_events)
+      HIP_TRY(launch::exec_row_evt(sch, ctx.G, a, n, ctx.threads, ctx.stream));
+    else if (ctx.cfg.row_ops)
+      HIP_TRY(launch::exec_row(sch, ctx.G, a, n, ctx.threads, ctx.stream));
+    else
+      HIP_TRY(launch::exec_evt(sch, ctx.G, a, n, ctx.threads, ctx.stream));
+    return true;
   }
   gcctb_ctx& ctx;
 };
@@ -248,7 +289,9 @@ class TwoPLHost : public SchemeHost {
  public:
   TwoPLHost(gcctb_ctx& c, bool wd) : SchemeHost(c), wait_die(wd) {}
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
-    HIP_TRY(launch::exec_tpl(wait_die, ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
+    const ExecArgs a = args(b, lo, hi);
+    if (launch_rows(a, hi - lo, persist)) return;
+    HIP_TRY(launch::exec_tpl(wait_die, ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
   // Locks are free again once every transaction of the batch committed; a
   // launch that stopped on an error is followed by clear_cc_state instead.
@@ -259,14 +302,18 @@ class TOHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
-    HIP_TRY(launch::exec_to(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
+    const ExecArgs a = args(b, lo, hi);
+    if (launch_rows(a, hi - lo, persist)) return;
+    HIP_TRY(launch::exec_to(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
 };
 class MVCCHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
-    HIP_TRY(launch::exec_mvcc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
+    const ExecArgs a = args(b, lo, hi);
+    if (launch_rows(a, hi - lo, persist)) return;
+    HIP_TRY(launch::exec_mvcc(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
   uint64_t GetMemSize() const override { return ctx.cc.n * 8 + ctx.vprev.n * 4 + ctx.pool.n * 8; }
   bool WantsProbePass() const override { return true; }
@@ -275,14 +322,18 @@ class SiloHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
-    HIP_TRY(launch::exec_silo(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
+    const ExecArgs a = args(b, lo, hi);
+    if (launch_rows(a, hi - lo, persist)) return;
+    HIP_TRY(launch::exec_silo(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
 };
 class TictocHost : public SchemeHost {
  public:
   using SchemeHost::SchemeHost;
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
-    HIP_TRY(launch::exec_tictoc(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
+    const ExecArgs a = args(b, lo, hi);
+    if (launch_rows(a, hi - lo, persist)) return;
+    HIP_TRY(launch::exec_tictoc(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
 };
 
@@ -303,6 +354,30 @@ class GaccoHost : public SchemeHost {
   void PostStream() override {}
   void PrepLaunch(uint64_t o0, uint64_t o1) override { prep_range(o0, o1); }
   bool ProbesRows() const override { return true; }
+  // Open launch: every row's queue starts empty; each admitted batch's
+  // accesses queue behind those of the batches admitted before it, in
+  // transaction-id order (rowbase counts the queued accesses per row).
+  void PrepOpen() override {
+    SchemeHost::PrepOpen();
+    HIP_TRY(launch::fill_u32(ctx.counters.p, ctx.counters.n, 0, ctx.stream));
+    HIP_TRY(launch::fill_u32(ctx.rowbase.p, ctx.rowbase.n, 0, ctx.stream));
+  }
+  void PrepAdmit(uint64_t o0, uint64_t o1, hipStream_t s) override {
+    const uint64_t n = o1 - o0;
+    if (n == 0) return;
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p + o0, n, ctx.rows_out.p + o0, s, ctx.sk0.p, ctx.sv0.p, (uint32_t)o0));
+    uint32_t *srow, *sop;
+    HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, bits_for(ctx.cfg.rows),
+                                   ctx.sort_tmp.p, &srow, &sop, s));
+    uint32_t* rflags = srow == ctx.sk0.p ? ctx.sk1.p : ctx.sk0.p;
+    uint32_t* rstart = sop == ctx.sv0.p ? ctx.sv1.p : ctx.sv0.p;
+    HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s, rflags));
+    HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
+    HIP_TRY(launch::inclusive_max_u32(rflags, rstart, n, ctx.sort_tmp.p, s));
+    HIP_TRY(launch::access_need_open(srow, sop, ctx.gstart.p, rstart, n, ctx.cfg.rows, ctx.rowbase.p, ctx.need.p, s));
+    HIP_TRY(launch::row_base_add(srow, rstart, n, ctx.cfg.rows, ctx.rowbase.p, s));
+  }
+  bool OpenRowsProbed() const override { return true; }
   // per-row access queues for ops [o0, o1) in (row, transaction id) order;
   // the rows also land in rows_out[o0, o1).  relative: turns count from the
   // row's first access and the counters start from one dense zero fill (the
@@ -335,7 +410,9 @@ class GaccoHost : public SchemeHost {
     HIP_TRY(launch::access_need(srow, sop, ctx.gstart.p, n, 0, ctx.need.p, ctx.counters.p, ctx.cfg.rows, s));
   }
   void Launch(int b, uint64_t lo, uint64_t hi, bool persist) override {
-    HIP_TRY(launch::exec_ordered(ctx.G, args(b, lo, hi), hi - lo, ctx.threads, persist, ctx.stream));
+    const ExecArgs a = args(b, lo, hi);
+    if (launch_rows(a, hi - lo, persist)) return;
+    HIP_TRY(launch::exec_ordered(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
   void Post(int, uint64_t, uint64_t) override {}
   uint64_t GetMemSize() const override { return ctx.counters.n * 4 + ctx.need.n * 4; }
@@ -455,6 +532,7 @@ class GPUTxHost : public SchemeHost {
     ExecArgs a = args(0, lo, hi);
     a.tx_map = ctx.tx_map.p;
     a.tx_rank = ctx.rank.p;
+    if (launch_rows(a, hi - lo, persist)) return;
     HIP_TRY(launch::exec_ordered(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
   void Post(int, uint64_t, uint64_t) override {}
@@ -546,12 +624,14 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   if (cfg->batch_size < 1) return set_error(GCCTB_EINVAL, "batch_size must be >= 1");
   if (cfg->rows < 1 || cfg->rows > 0xfffffffeull) return set_error(GCCTB_EINVAL, "rows out of range");
   if (cfg->cols < 1 || cfg->cols > 255) return set_error(GCCTB_EINVAL, "cols out of range");
-  // The adaptive admission window is switched off: with Silo it made a kernel
-  // access memory out of bounds (and once return reads that differ from the
-  // serial replay) in runs whose cause has not been found.
+  // The adaptive admission window is not part of this build: with Silo it
+  // once made a kernel access memory out of bounds (and once return reads
+  // that differ from the serial replay), its cause was never shown, and its
+  // code was taken out of the kernels.
   if (cfg->persist_adaptive)
-    return set_error(GCCTB_EINVAL, "persist_adaptive is disabled in this build (a memory fault with Silo is not "
-                                   "resolved); use a static persist_window");
+    return set_error(GCCTB_EINVAL, "persist_adaptive is not supported in this build; use a static persist_window");
+  if (cfg->row_ops && cfg->cols > (uint32_t)ROW_MAX_COLS)
+    return set_error(GCCTB_EINVAL, "whole-row mode takes rows of at most 16 cells");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_error(GCCTB_EDEVICE, "no HIP device: the transaction kernels need an MI355X (gfx950)");
@@ -622,9 +702,11 @@ void gcctb_destroy(gcctb_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->h_err) (void)hipHostFree(c->h_err);
-  hipStream_t s = c->stream;
+  hipStream_t s = c->stream, s2 = c->stream2;
+  if (s2) (void)hipStreamSynchronize(s2);
   delete c;  // DevBuf destructors free device memory
   if (s) (void)hipStreamDestroy(s);
+  if (s2) (void)hipStreamDestroy(s2);
 }
 
 int gcctb_build_index_identity(gcctb_ctx* c) {
@@ -795,9 +877,16 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
     HIP_TRY(hipMemsetAsync(c->reads.p, 0, c->reads.n * sizeof(uint64_t), c->stream));
     HIP_TRY(hipMemsetAsync(c->tx_aborts.p, 0, c->tx_aborts.n * sizeof(uint32_t), c->stream));
     HIP_TRY(launch::op_tx(c->tx_off.p, ntx, c->op_tx.p, c->stream));
+    if (c->cfg.debug_events) {
+      c->ev_op.alloc(nops);
+      c->ev_commit.alloc(ntx);
+      c->events_ready = false;
+    }
     if (c->scheme == SCHEME_MVCC) {
+      // node: ts word, link, the superseded cell (or row, in whole-row mode)
+      const uint64_t stride = c->cfg.row_ops ? 2 + (uint64_t)c->cfg.cols : 3;
       c->pool_cap = cap;
-      c->pool.alloc(3 * cap + 3);
+      c->pool.alloc(stride * cap + stride);
       // nodes are published before they are linked, but a zeroed pool keeps
       // an unpublished node from ever holding a link that points anywhere
       HIP_TRY(hipMemsetAsync(c->pool.p, 0, c->pool.n * sizeof(uint64_t), c->stream));
@@ -835,12 +924,6 @@ enum RunForm { FORM_PLAIN = 0, FORM_GRAPH = 1, FORM_PERSIST = 2 };
 // Largest per-batch sequence the graph form captures (GPUTx inputs can have
 // one rank level per launch by the hundred thousand).
 constexpr uint64_t GRAPH_MAX_LAUNCHES = 4096;
-
-// Slot counter and first admission window of a persistent launch: the
-// configured window, or every group of the grid.
-static uint64_t initial_window(const gcctb_ctx* c) {
-  return c->cfg.persist_window ? (uint64_t)c->cfg.persist_window : ~0ull >> 1;
-}
 
 // Totals of the per-slot metrics the kernels accumulated since the last clear.
 static Metrics collect_metrics(gcctb_ctx* c) {
@@ -881,11 +964,19 @@ static std::string error_text(uint64_t flags) {
 static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
   if (!c) return set_error(GCCTB_EINVAL, "null context");
   if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
+  if (form == FORM_PERSIST && (c->cfg.row_ops || c->cfg.debug_events))
+    return set_error(GCCTB_EINVAL, "whole-row mode and the event log run the plain and graph forms");
   gcctb_stats st{};
   Metrics m{};
   try {
     hipStream_t s = c->stream;
     HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics) * METRIC_SLOTS, s));
+    if (c->cfg.debug_events) {
+      // a fresh log per run: no stamps, the stamp counter at zero
+      if (c->nops) HIP_TRY(launch::fill_u64(c->ev_op.p, c->nops, 0, s));
+      if (c->ntx) HIP_TRY(launch::fill_u64(c->ev_commit.p, c->ntx, 0, s));
+      HIP_TRY(launch::fill_u64(c->ctrs.p + CTR_EVENT, 1, 0, s));
+    }
     HIP_TRY(hipStreamSynchronize(s));
     *c->h_err = 0;
     float ms = 0;
@@ -901,7 +992,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       const bool probed = c->host->ProbesRows() || pass;
       if (!c->host->ProbesRows() && pass && c->nops)
         HIP_TRY(launch::probe(c->ix, c->ops.p, c->nops, c->rows_out.p, s));
-      HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
+      HIP_TRY(launch::persist_begin(c->ctrs.p, s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
       c->rows_probed = probed;
       c->host->Launch(0, 0, c->ntx, true);
@@ -1042,6 +1133,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
   st.total_ops = c->nops;
   st.read_ops = c->nreads;
   c->last_committed = m.committed;
+  c->events_ready = c->cfg.debug_events != 0;
   if (out) *out = st;
   if (m.error_flags) return set_error(GCCTB_EKERNEL, error_text(m.error_flags));
   if (m.committed != c->ntx) return set_error(GCCTB_EKERNEL, "not every transaction committed");
@@ -1053,17 +1145,24 @@ int gcctb_run_graph(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FO
 int gcctb_run_persistent(gcctb_ctx* c, gcctb_stats* out) { return run_form(c, out, FORM_PERSIST); }
 
 // ---------------------------------------------------------------------------
-// Serving form.  Requests arrive as batches at given times; table, index and
-// CC state stay resident on the device.  Whenever the device is idle the host
-// admits every batch that has arrived, up to the admission limits, copies
-// their ops from pinned host memory and runs them as one persistent launch
-// (probe pass, persistent kernel, restore of the touched CC words).  A
-// transaction's latency is its launch's completion time minus its batch's
-// arrival time, both on the host clock.
-// Admission: at most max_launch_txs transactions per launch, and, for MVCC,
-// at most as many writes as the version pool holds (the pool budget is
-// pool_cap_bytes, 24 bytes per version).  Batches that arrived but do not fit
-// wait for the next launch (counted in `deferred`).
+// Serving form: continuous batching.  Requests arrive as batches at given
+// times; table, index and CC state stay resident on the device.  One open
+// persistent launch runs while requests arrive.  The host admits each batch
+// as it arrives: on the admission stream it copies the batch's ops and
+// offsets from pinned host memory into device memory, prepares them (the
+// probe pass for MVCC; GaccO's per-row turns), then raises the launch's
+// admission word, and the launch's groups take the admitted slots as they
+// free up, next to the transactions still running.  A launch is closed (its
+// admission word gets the closed bit; it drains and its touched CC words are
+// restored) and a new one opened when the next batch would exceed the
+// admission limits: max_launch_txs transactions per launch, and, for MVCC,
+// the version pool (pool_cap_bytes at 24 bytes per version, bounded by the
+// allocated pool and by free device memory), whose versions stay live until
+// their launch ends.  A launch is also closed before an idle gap longer than
+// a quarter of the watchdog period (its waiting waves would otherwise reach
+// the watchdog).  Latency: a transaction's commit stamp (device wall clock,
+// mapped to host time through a clock sample taken at the start) minus its
+// batch's arrival.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1081,6 +1180,8 @@ double percentile(std::vector<double> v, double q) {
   return v[k];
 }
 
+constexpr int ADMIT_RING = 64;  // staging blocks of admission words in flight
+
 }  // namespace
 
 int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batch_sizes, const double* arrival_ms,
@@ -1089,6 +1190,8 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
   if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
   if (c->scheme == SCHEME_GPUTX)
     return set_error(GCCTB_EINVAL, "gputx ranks its whole input before running; it has no serving form");
+  if (c->cfg.row_ops || c->cfg.debug_events)
+    return set_error(GCCTB_EINVAL, "whole-row mode and the event log run the plain and graph forms");
   uint64_t total = 0;
   for (uint64_t i = 0; i < nbatches; ++i) {
     if (batch_sizes[i] == 0) return set_error(GCCTB_EINVAL, "empty batch");
@@ -1100,115 +1203,197 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
   const uint64_t max_txs = sc->max_launch_txs ? sc->max_launch_txs : c->ntx;
   uint64_t pool_nodes = ~0ull;
   if (c->scheme == SCHEME_MVCC) {
+    // admission by memory: the allocated pool, the configured budget, and
+    // what free device memory would hold besides the allocated pool
     pool_nodes = c->pool_cap;
     if (sc->pool_cap_bytes) pool_nodes = std::min(pool_nodes, sc->pool_cap_bytes / 24);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      pool_nodes = std::min<uint64_t>(pool_nodes, ((uint64_t)free_b + c->pool.n * 8) / 24);
   }
   gcctb_serve_stats st{};
   st.txs = c->ntx;
+  st.pool_nodes = pool_nodes;
   using clk = std::chrono::steady_clock;
   const uint64_t saved_pool_cap = c->pool_cap;
+  std::vector<hipEvent_t> ring_ev(ADMIT_RING, nullptr);
+  hipEvent_t ev_open = nullptr, ev_done = nullptr, ev_s2 = nullptr;
+  bool launch_open = false;  // an open launch is running (closed on the error path too)
+  auto cleanup = [&]() {
+    c->open_launch = false;
+    c->rows_probed = false;
+    c->record_commit = false;
+    c->pool_cap = saved_pool_cap;
+    for (auto& e : ring_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (ev_open) (void)hipEventDestroy(ev_open);
+    if (ev_done) (void)hipEventDestroy(ev_done);
+    if (ev_s2) (void)hipEventDestroy(ev_s2);
+  };
   try {
     hipStream_t s = c->stream;
-    // requests wait in pinned host memory until admitted
-    PinnedBuf host_ops;
+    if (!c->stream2) HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    hipStream_t s2 = c->stream2;
+    for (auto& e : ring_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev_open, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev_s2, hipEventDisableTiming));
+    std::vector<bool> ring_used(ADMIT_RING, false);
+    // requests wait in pinned host memory until admitted: ops and offsets
+    PinnedBuf host_ops, host_off, staging;
     if (c->nops) {
       HIP_TRY(hipHostMalloc((void**)&host_ops.p, c->nops * 8, hipHostMallocDefault));
       HIP_TRY(hipMemcpy(host_ops.p, c->ops.p, c->nops * 8, hipMemcpyDeviceToHost));
     }
-    // per-batch write counts (version-pool admission)
+    HIP_TRY(hipHostMalloc((void**)&host_off.p, (c->ntx + 1) * 8, hipHostMallocDefault));
+    std::memcpy(host_off.p, c->h_tx_off.data(), (c->ntx + 1) * 8);
+    constexpr uint64_t ADMIT_WORDS = 16 * GATE_SHARDS;
+    HIP_TRY(hipHostMalloc((void**)&staging.p, ADMIT_RING * ADMIT_WORDS * 8, hipHostMallocDefault));
+    std::memset(staging.p, 0, ADMIT_RING * ADMIT_WORDS * 8);
+    if (c->scheme == SCHEME_GACCO) c->rowbase.alloc(c->cfg.rows);
+    c->clock_word.alloc(1);
+    // per-batch first transactions and write counts (version-pool admission)
     std::vector<uint64_t> bfirst(nbatches + 1, 0), bwrites(nbatches, 0);
     for (uint64_t i = 0; i < nbatches; ++i) {
       bfirst[i + 1] = bfirst[i] + batch_sizes[i];
       for (uint64_t o = c->h_tx_off[bfirst[i]]; o < c->h_tx_off[bfirst[i + 1]]; ++o)
         bwrites[i] += op_is_write(host_ops.p[o]) ? 1 : 0;
-      if (bwrites[i] > pool_nodes) return set_error(GCCTB_EINVAL, "a batch has more writes than the version pool holds");
+      if (bwrites[i] > pool_nodes) {
+        cleanup();
+        return set_error(GCCTB_EINVAL, "a batch has more writes than the version pool holds");
+      }
     }
     HIP_TRY(hipMemsetAsync(c->metrics.p, 0, sizeof(Metrics) * METRIC_SLOTS, s));
     HIP_TRY(hipStreamSynchronize(s));
     *c->h_err = 0;
     if (c->scheme == SCHEME_MVCC) c->pool_cap = std::min(c->pool_cap, pool_nodes);
-    std::vector<double> lat(c->ntx, 0.0);
-    c->tcommit.alloc(c->ntx);
+    c->tcommit.alloc(std::max<uint64_t>(1, c->ntx));
+    HIP_TRY(launch::fill_u64(c->tcommit.p, c->tcommit.n, 0, s));
     c->record_commit = true;
-    struct LaunchRec {
-      uint64_t lo, hi, b0, b1;
-      double done_ms;
-    };
-    std::vector<LaunchRec> recs;
-    double busy = 0, launch_txs = 0;
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+    const double ticks_per_ms = khz > 0 ? (double)khz : 1e5;
+    const double idle_close_ms = c->watchdog_ticks ? 0.25 * (double)c->watchdog_ticks / ticks_per_ms : 1e300;
     const auto t0 = clk::now();
     auto since = [&](clk::time_point t) { return std::chrono::duration<double, std::milli>(t - t0).count(); };
-    uint64_t next = 0, launches = 0;
-    while (next < nbatches) {
+    // clock sample: device tick d0 at host time h0 (midpoint of the round trip)
+    double h0 = 0;
+    uint64_t d0 = 0;
+    {
+      const auto a0 = clk::now();
+      HIP_TRY(launch::clock_sample(c->clock_word.p, s));
+      HIP_TRY(hipMemcpyAsync(&d0, c->clock_word.p, 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      const auto a1 = clk::now();
+      h0 = 0.5 * (since(a0) + since(a1));
+    }
+    std::vector<double> admit_ms(nbatches, 0.0);
+    // the open launch
+    struct Session {
+      bool open = false;
+      uint64_t lo = 0, hi = 0, writes = 0;
+      double start_ms = 0;
+    } sess;
+    uint64_t ring_pos = 0, launches = 0;
+    double busy = 0, launch_txs = 0;
+    auto write_admit = [&](uint32_t value) {
+      const uint64_t k = ring_pos++ % ADMIT_RING;
+      if (ring_used[k]) HIP_TRY(hipEventSynchronize(ring_ev[k]));  // its last copy has read the block
+      uint64_t* blk = staging.p + k * ADMIT_WORDS;
+      for (int g = 0; g < GATE_SHARDS; ++g) blk[16 * g] = value;  // the u32 word (low half), zero above
+      HIP_TRY(hipMemcpyAsync(c->ctrs.p + CTR_ADMIT0, blk, ADMIT_WORDS * 8, hipMemcpyHostToDevice, s2));
+      HIP_TRY(hipEventRecord(ring_ev[k], s2));
+      ring_used[k] = true;
+    };
+    auto open_session = [&](uint64_t lo) {
+      c->host->PrepOpen();
+      // the admission stream starts on this launch only after its reset
+      HIP_TRY(hipEventRecord(ev_open, s));
+      HIP_TRY(hipStreamWaitEvent(s2, ev_open, 0));
+      c->open_launch = true;
+      c->rows_probed = c->host->OpenRowsProbed();
+      c->host->Launch((int)launches, lo, c->ntx, true);  // capacity: every remaining slot
+      c->open_launch = false;
+      c->rows_probed = false;
+      sess.open = launch_open = true;
+      sess.lo = sess.hi = lo;
+      sess.writes = 0;
+      sess.start_ms = since(clk::now());
+      ++launches;
+    };
+    auto close_session = [&]() {
+      write_admit((uint32_t)(sess.hi - sess.lo) | ADMIT_CLOSED);
+      // the restore reads rows the admission stream wrote: order it after
+      // that stream's work
+      HIP_TRY(hipEventRecord(ev_s2, s2));
+      HIP_TRY(hipStreamWaitEvent(s, ev_s2, 0));
+      c->host->Post((int)launches - 1, sess.lo, sess.hi - sess.lo);
+      enqueue_error_copy(*c);
+      HIP_TRY(hipEventRecord(ev_done, s));
+      HIP_TRY(hipEventSynchronize(ev_done));
+      busy += since(clk::now()) - sess.start_ms;
+      launch_txs += (double)(sess.hi - sess.lo);
+      sess.open = launch_open = false;
+    };
+    uint64_t next = 0;
+    while (next < nbatches && !*c->h_err) {
       const double due = arrival_ms ? arrival_ms[next] : 0.0;
       double now = since(clk::now());
       if (due > now) {
+        // a long idle gap: close the open launch instead of leaving it waiting
+        if (sess.open && due - now > idle_close_ms) {
+          close_session();
+          if (*c->h_err) break;
+          now = since(clk::now());
+        }
         if (due - now > 0.2) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)((due - now - 0.1) * 1e3)));
         while (since(clk::now()) < due) {
         }
         now = since(clk::now());
       }
-      // admit every arrived batch that fits
-      uint64_t end = next, txs = 0, writes = 0;
-      while (end < nbatches && (!arrival_ms || arrival_ms[end] <= now)) {
-        if (end > next && (txs + batch_sizes[end] > max_txs || writes + bwrites[end] > pool_nodes)) {
+      // admit every batch that has arrived
+      while (next < nbatches && (!arrival_ms || arrival_ms[next] <= now)) {
+        const uint64_t n = batch_sizes[next];
+        if (sess.open && (sess.hi - sess.lo + n > max_txs || sess.writes + bwrites[next] > pool_nodes)) {
+          close_session();  // the batch waits for the running launch to drain
           st.deferred++;
-          break;
+          if (*c->h_err) break;
         }
-        txs += batch_sizes[end];
-        writes += bwrites[end];
-        ++end;
+        if (!sess.open)
+          open_session(bfirst[next]);
+        else
+          st.admitted_live++;  // joins a launch that is already running
+        const uint64_t lo2 = bfirst[next], hi2 = bfirst[next + 1];
+        const uint64_t o0 = c->h_tx_off[lo2], o1 = c->h_tx_off[hi2];
+        if (o1 > o0) HIP_TRY(hipMemcpyAsync(c->ops.p + o0, host_ops.p + o0, (o1 - o0) * 8, hipMemcpyHostToDevice, s2));
+        HIP_TRY(hipMemcpyAsync(c->tx_off.p + lo2, host_off.p + lo2, (hi2 - lo2 + 1) * 8, hipMemcpyHostToDevice, s2));
+        c->host->PrepAdmit(o0, o1, s2);
+        sess.hi = hi2;
+        sess.writes += bwrites[next];
+        write_admit((uint32_t)(sess.hi - sess.lo));
+        admit_ms[next] = since(clk::now());
+        ++next;
+        now = since(clk::now());
       }
-      const uint64_t lo = bfirst[next], hi = bfirst[end];
-      const uint64_t o0 = c->h_tx_off[lo], o1 = c->h_tx_off[hi];
-      const auto tl = clk::now();
-      if (o1 > o0) HIP_TRY(hipMemcpyAsync(c->ops.p + o0, host_ops.p + o0, (o1 - o0) * 8, hipMemcpyHostToDevice, s));
-      c->host->PrepLaunch(o0, o1);
-      const bool pass = c->probe_pass >= 0 ? c->probe_pass != 0 : c->host->WantsProbePass();
-      if (o1 > o0 && !c->host->ProbesRows() && pass)
-        HIP_TRY(launch::probe(c->ix, c->ops.p + o0, o1 - o0, c->rows_out.p + o0, s));
-      HIP_TRY(launch::adapt_begin(c->ctrs.p, initial_window(c), s));
-      c->rows_probed = c->host->ProbesRows() || pass;
-      c->host->Launch((int)launches, lo, hi, true);
-      c->rows_probed = false;
-      c->host->Post((int)launches, lo, hi - lo);
-      enqueue_error_copy(*c);
-      HIP_TRY(hipEventRecord(c->ev[7], s));
-      HIP_TRY(hipEventSynchronize(c->ev[7]));
-      const auto td = clk::now();
-      const double done = since(td);
-      busy += std::chrono::duration<double, std::milli>(td - tl).count();
-      recs.push_back({lo, hi, next, end, done});
-      launch_txs += (double)txs;
-      ++launches;
-      next = end;
-      if (*c->h_err) break;  // a launch that reported an error ends the loop
     }
+    if (sess.open) close_session();
     st.tot_ms = since(clk::now());
-    c->record_commit = false;
-    c->pool_cap = saved_pool_cap;
-    // Commit latency.  The kernels stamp each commit with the device wall
-    // clock; a launch's last commit happened no later than the host saw the
-    // launch complete, so ticks map to host time through that completion
-    // (the mapping can only make a commit look later, by the restore kernel
-    // and the completion notice).
+    // commit latency: stamp mapped to host time, never before the admission
+    std::vector<double> lat(c->ntx, 0.0);
     {
       std::vector<uint64_t> tick(c->ntx);
       if (c->ntx) HIP_TRY(hipMemcpy(tick.data(), c->tcommit.p, c->ntx * 8, hipMemcpyDeviceToHost));
-      int khz = 0;
-      HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
-      const double ticks_per_ms = khz > 0 ? (double)khz : 1e5;
-      for (const LaunchRec& r : recs) {
-        uint64_t last = 0;
-        for (uint64_t t = r.lo; t < r.hi; ++t) last = std::max(last, tick[t]);
-        for (uint64_t b = r.b0; b < r.b1; ++b) {
-          const double arr = arrival_ms ? arrival_ms[b] : 0.0;
-          for (uint64_t t = bfirst[b]; t < bfirst[b + 1]; ++t)
-            lat[t] = r.done_ms - (double)(last - tick[t]) / ticks_per_ms - arr;
+      for (uint64_t b = 0; b < nbatches; ++b) {
+        const double arr = arrival_ms ? arrival_ms[b] : 0.0;
+        for (uint64_t t = bfirst[b]; t < bfirst[b + 1]; ++t) {
+          const double commit = h0 + ((double)(int64_t)(tick[t] - d0)) / ticks_per_ms;
+          lat[t] = std::max(commit, admit_ms[b]) - arr;
         }
       }
     }
     const Metrics m = collect_metrics(c);
+    c->record_commit = false;
+    c->pool_cap = saved_pool_cap;
     if (m.error_flags) clear_cc_state(*c);
     st.committed = m.committed;
     st.aborts = m.abort;
@@ -1229,11 +1414,25 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
     if (tx_latency_ms && !lat.empty()) std::memcpy(tx_latency_ms, lat.data(), lat.size() * sizeof(double));
     c->last_committed = m.committed;
   } catch (const HipFailure& f) {
-    c->rows_probed = false;
-    c->record_commit = false;
-    c->pool_cap = saved_pool_cap;
+    if (launch_open) {
+      // close the running launch so that it drains instead of waiting for
+      // the watchdog: every admission word gets the closed bit at the
+      // admitted count it already holds
+      uint64_t words[16 * GATE_SHARDS];
+      if (hipMemcpy(words, c->ctrs.p + CTR_ADMIT0, sizeof(words), hipMemcpyDeviceToHost) == hipSuccess) {
+        for (int g = 0; g < GATE_SHARDS; ++g) words[16 * g] |= (uint64_t)ADMIT_CLOSED;  // the u32's top bit
+        (void)hipMemcpy(c->ctrs.p + CTR_ADMIT0, words, sizeof(words), hipMemcpyHostToDevice);
+      }
+      (void)hipStreamSynchronize(c->stream);
+      try {
+        clear_cc_state(*c);
+      } catch (const HipFailure&) {
+      }
+    }
+    cleanup();
     return hip_fail(f);
   }
+  cleanup();
   if (out) *out = st;
   if (st.error_flags) return set_error(GCCTB_EKERNEL, error_text(st.error_flags));
   if (st.committed != c->ntx) return set_error(GCCTB_EKERNEL, "not every transaction committed");
@@ -1354,10 +1553,50 @@ int gcctb_check(gcctb_ctx* c, const uint64_t* init_table, uint64_t* bad, uint64_
   }
   const int rc = gcctb_fetch(c, nullptr, nullptr, order.data(), nullptr);
   if (rc != GCCTB_OK) return rc;
-  check_history(c->cfg.rows, c->cfg.cols, c->cfg.seed, init_table, fin.data(), ops.data(), rows.data(),
-                c->h_tx_off.data(), c->ntx, order.data(), reads.data(), bad);
+  check_history(c->cfg.rows, c->cfg.cols, c->cfg.seed, c->cfg.row_ops != 0, init_table, fin.data(), ops.data(),
+                rows.data(), c->h_tx_off.data(), c->ntx, order.data(), reads.data(), bad);
   if (uncommitted) *uncommitted = c->ntx - std::min(c->ntx, c->last_committed);
   return GCCTB_OK;
+}
+
+int gcctb_fetch_events(gcctb_ctx* c, gcctb_event* events, uint64_t cap, uint64_t* n) {
+  if (!c || !n || (cap && !events)) return set_error(GCCTB_EINVAL, "null argument");
+  if (!c->cfg.debug_events || !c->events_ready) return set_error(GCCTB_ESTATE, "no logged run (create with debug_events = 1)");
+  try {
+    std::vector<uint64_t> evo(c->nops), evc(c->ntx), ops(c->nops);
+    std::vector<uint32_t> rows(c->nops);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->nops) {
+      HIP_TRY(hipMemcpy(evo.data(), c->ev_op.p, c->nops * 8, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(ops.data(), c->ops.p, c->nops * 8, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(rows.data(), c->rows_out.p, c->nops * 4, hipMemcpyDeviceToHost));
+    }
+    if (c->ntx) HIP_TRY(hipMemcpy(evc.data(), c->ev_commit.p, c->ntx * 8, hipMemcpyDeviceToHost));
+    uint64_t k = 0;
+    for (uint64_t t = 0; t < c->ntx; ++t) {
+      for (uint64_t i = c->h_tx_off[t]; i < c->h_tx_off[t + 1]; ++i) {
+        if (!evo[i]) continue;
+        if (k < cap) events[k] = gcctb_event{evo[i], rows[i], (int32_t)t, op_is_write(ops[i]) ? 1 : 0};
+        ++k;
+      }
+      if (evc[t]) {
+        if (k < cap) events[k] = gcctb_event{evc[t], 0, (int32_t)t, 2};
+        ++k;
+      }
+    }
+    *n = k;
+    if (k > cap) return set_error(GCCTB_EINVAL, "the event buffer is too small");
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  return GCCTB_OK;
+}
+
+int gcctb_conflict_check(const gcctb_event* events, uint64_t n, uint64_t ntx, uint64_t* uncommitted, uint64_t* cycle,
+                         uint64_t cycle_cap, uint64_t* cycle_len, char* report, uint64_t report_len) {
+  if ((n && !events) || (cycle_cap && !cycle)) return set_error(GCCTB_EINVAL, "null argument");
+  if (ntx >= (1ull << 31)) return set_error(GCCTB_EINVAL, "too many transactions");
+  return conflict_check(events, n, ntx, uncommitted, cycle, cycle_cap, cycle_len, report, report_len);
 }
 
 int gcctb_sort_pairs(void* keys, uint32_t* vals, uint64_t n, int32_t key_bytes, int32_t key_bits, double* ms) {
@@ -1418,8 +1657,10 @@ int gcctb_sort_pairs(void* keys, uint32_t* vals, uint64_t n, int32_t key_bytes, 
 }
 
 int gcctb_microbench(int32_t which, uint64_t n_words, uint64_t threads, uint64_t iters, double* rate, double* ms) {
-  if (which < 0 || which > 9 || n_words < 16 || threads == 0 || iters == 0)
+  if (which < 0 || which > 13 || n_words < 16 || threads == 0 || iters == 0)
     return set_error(GCCTB_EINVAL, "bad microbench arguments");
+  // the hot-word modes address words up to 15 * 16 + 15
+  if (which >= 10 && n_words < 256) return set_error(GCCTB_EINVAL, "hot-word microbench modes need n_words >= 256");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(GCCTB_EDEVICE, "no HIP device");
   const hipError_t e = launch::microbench(which, n_words, threads, iters, rate, ms);

@@ -17,8 +17,13 @@ int write_txs_file(const char* path, const uint64_t* ops, const uint64_t* tx_off
 }  // namespace gcctb
 
 namespace gcctb {
-int check_history(uint64_t rows, uint32_t cols, uint64_t seed, const uint64_t* init_table,
+int check_history(uint64_t rows, uint32_t cols, uint64_t seed, bool row_ops, const uint64_t* init_table,
                   const uint64_t* final_table, const uint64_t* ops, const uint32_t* op_rows,
                   const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, const uint64_t* reads,
                   uint64_t* bad);
+}  // namespace gcctb
+
+namespace gcctb {
+int conflict_check(const gcctb_event* ev, uint64_t n, uint64_t ntx, uint64_t* uncommitted, uint64_t* cycle,
+                   uint64_t cycle_cap, uint64_t* cycle_len, char* report, uint64_t report_len);
 }  // namespace gcctb

@@ -3,14 +3,21 @@
 //   test_ycsb <dataset> <rownum> <txcnt> <cc_scheme> <warp_density> <block_size> <batch_size> <debug>
 //
 // <dataset> is a reference .txs file (raw YCSBTx records, include/ycsb.cuh) or
-// "gen:theta=<t>,rr=<r>,len=<l>[,lenmax=<m>]" for a generated workload.  The
-// table has <rownum> rows of 10 seeded u64 columns with a sorted-array index on
-// the id; the run uses the reference's launch structure.  Output: the same
+// "gen:theta=<t>,rr=<r>,len=<l>[,lenmax=<m>][,row=1]" for a generated
+// workload.  A .txs file runs like the reference: rows of 13 seeded u64 cells
+// (the 104-byte Item) and every op reading or writing its whole row
+// (benchmark/ycsb.cu:57,86-91).  A generated workload runs on rows of 10 u64
+// cells with one cell per op, or whole rows of 13 cells with row=1.  The
+// index maps the id to the row; the run uses the reference's launch
+// structure.  Output: the same
 // stdout block and 15-field CSV line (appended to <dataset>.txt) as
-// ExecInfo::Analyse (include/runtime.cuh:169-214).  With debug != 0 the run is
-// checked afterwards: every read must return the latest preceding write in the
-// serialization order the kernels reported, and every transaction must have
-// committed (the reference's check, src/runtime.cu:129-158, prints GOOD/BAD).
+// ExecInfo::Analyse (include/runtime.cuh:169-214).  With debug != 0 the kernels
+// record the event log (AddEvent, include/runtime.cuh:64-81) and the run is
+// checked afterwards like the reference's ExecInfo::Check
+// (src/runtime.cu:23-158): conflict graph over the events, UNCOMMITTED
+// transactions, a cycle printed with its transactions and events, GOOD or
+// BAD; then every read must return the latest preceding write in the
+// serialization order the kernels reported (REPLAY OK / REPLAY BAD).
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -24,12 +31,14 @@
 
 static const uint64_t SEED = 141919810;  // benchmark_cpu/ycsb.cu:27
 
-static bool parse_gen(const std::string& spec, double& theta, double& rr, uint32_t& len, uint32_t& lenmax) {
+static bool parse_gen(const std::string& spec, double& theta, double& rr, uint32_t& len, uint32_t& lenmax,
+                      int& row_ops) {
   if (spec.rfind("gen:", 0) != 0) return false;
   theta = 0.0;
   rr = 0.9;
   len = 16;
   lenmax = 0;
+  row_ops = 0;
   std::string rest = spec.substr(4);
   size_t pos = 0;
   while (pos < rest.size()) {
@@ -43,6 +52,7 @@ static bool parse_gen(const std::string& spec, double& theta, double& rr, uint32
       if (k == "rr") rr = std::atof(v.c_str());
       if (k == "len") len = (uint32_t)std::atoi(v.c_str());
       if (k == "lenmax") lenmax = (uint32_t)std::atoi(v.c_str());
+      if (k == "row") row_ops = std::atoi(v.c_str()) != 0 ? 1 : 0;
     }
     pos = end + 1;
   }
@@ -78,10 +88,13 @@ int main(int argc, char** argv) {
   uint64_t nops = 0;
   double theta, rr;
   uint32_t len, lenmax;
-  if (parse_gen(argv[1], theta, rr, len, lenmax)) {
+  int row_ops = 1;     // a .txs file: whole-row ops on 104-byte rows
+  uint32_t cols = 13;
+  if (parse_gen(argv[1], theta, rr, len, lenmax, row_ops)) {
+    cols = row_ops ? 13 : 10;
     gcctb_workload w{};
     w.rows = (uint64_t)rownum;
-    w.cols = 10;
+    w.cols = cols;
     w.ntx = (uint64_t)txcnt;
     w.len_min = len;
     w.len_max = lenmax;
@@ -92,7 +105,7 @@ int main(int argc, char** argv) {
     if (gcctb_generate(&w, ops.data(), off.data(), &nops) != GCCTB_OK) return die("generate");
   } else {
     ops.resize((size_t)txcnt * 16 + 1);
-    if (gcctb_read_txs_file(argv[1], (uint64_t)txcnt, 10, SEED, ops.data(), off.data(), &nops) != GCCTB_OK)
+    if (gcctb_read_txs_file(argv[1], (uint64_t)txcnt, cols, SEED, ops.data(), off.data(), &nops) != GCCTB_OK)
       return die("read dataset");
   }
 
@@ -102,15 +115,17 @@ int main(int argc, char** argv) {
   cfg.block_size = warp_cnt;
   cfg.batch_size = (uint64_t)batch_size;
   cfg.rows = (uint64_t)rownum;
-  cfg.cols = 10;
+  cfg.cols = cols;
   cfg.seed = SEED;
   cfg.backoff_ticks = 20;
+  cfg.row_ops = row_ops;
+  cfg.debug_events = debug ? 1 : 0;
   gcctb_ctx* ctx = nullptr;
   if (gcctb_create(&cfg, &ctx) != GCCTB_OK) return die("create");
   if (gcctb_seed_table(ctx) != GCCTB_OK) return die("seed table");
   std::vector<uint64_t> init;
   if (debug) {
-    init.resize((size_t)rownum * 10);
+    init.resize((size_t)rownum * cols);
     if (gcctb_fetch(ctx, init.data(), nullptr, nullptr, nullptr) != GCCTB_OK) return die("fetch");
   }
   if (gcctb_load_ops(ctx, ops.data(), off.data(), (uint64_t)txcnt) != GCCTB_OK) return die("load ops");
@@ -155,12 +170,19 @@ int main(int argc, char** argv) {
 
   int status = 0;
   if (debug) {
+    std::vector<gcctb_event> events(nops + (uint64_t)txcnt + 1);
+    uint64_t nev = 0, unc = 0, cycle_len = 0;
+    if (gcctb_fetch_events(ctx, events.data(), events.size(), &nev) != GCCTB_OK) return die("fetch events");
+    std::vector<char> report(1 << 20);
+    if (gcctb_conflict_check(events.data(), nev, (uint64_t)txcnt, &unc, nullptr, 0, &cycle_len, report.data(),
+                             report.size()) != GCCTB_OK)
+      return die("conflict check");
+    std::cout << report.data();
     uint64_t bad = 0, uncommitted = 0;
     if (gcctb_check(ctx, init.data(), &bad, &uncommitted) != GCCTB_OK) return die("check");
-    std::cout << "SCAN OK\n";
-    if (uncommitted) std::cout << "UNCOMMITTED " << uncommitted << "\n";
-    std::cout << ((bad || uncommitted) ? "BAD\n" : "GOOD\n");
-    status = (bad || uncommitted) ? 2 : 0;
+    if (bad) std::cout << "REPLAY BAD " << bad << "\n";
+    else std::cout << "REPLAY OK\n";
+    status = (cycle_len || unc || bad || uncommitted) ? 2 : 0;
   }
   gcctb_destroy(ctx);
   return status;

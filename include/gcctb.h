@@ -45,9 +45,15 @@ typedef struct gcctb_config {
                               is the failed attempt's duration when that is longer */
   uint32_t persist_window; /* persistent and serving forms: at most this many transactions in
                               flight (0 = as many as the device holds) */
-  int32_t persist_adaptive; /* 1: the window adapts to the abort rate while the launch runs,
-                               starting from persist_window.  Disabled in this build: gcctb_create
-                               returns GCCTB_EINVAL for 1 */
+  int32_t persist_adaptive; /* not supported in this build: gcctb_create returns GCCTB_EINVAL
+                               for 1 (the abort-rate-driven window was taken out) */
+  int32_t row_ops;          /* 1: whole-row mode, the reference's Item copies
+                               (benchmark/ycsb.cu:57,86-91): every op reads or writes all cols
+                               cells of its row (cols <= 16; 13 u64 = the 104-byte Item), a
+                               written cell c holds mix64(write value + c), and a read returns
+                               the row's digest; plain and graph forms only */
+  int32_t debug_events;     /* 1: the plain and graph forms record the event log of
+                               gcctb_fetch_events (the debug mode's AddEvent) */
 } gcctb_config;
 
 /* Result of one run.  The first block mirrors the reference's Metrics and
@@ -142,14 +148,19 @@ int gcctb_run_persistent(gcctb_ctx* ctx, gcctb_stats* out);
  * file, benchmark_cpu/ycsb.cu:242-291).  The loaded transactions are requests
  * that arrive in batches: batch i holds the next batch_sizes[i] transactions
  * and arrives arrival_ms[i] after the call starts (null: every batch at 0, the
- * peak-load case).  Table, index and CC state stay on the device; whenever it
- * is idle, every arrived batch that fits the admission limits is copied from
- * pinned host memory and run as one persistent launch.  tx_latency_ms (may be
- * null) receives, per transaction, its commit time minus the arrival of its
- * batch (commits are stamped with the device wall clock and mapped to host
- * time through the launch's completion).  Order keys from gcctb_fetch are
- * {launch, key0, key1}.  Not
- * available for gputx (it ranks its whole input before running). */
+ * peak-load case).  Table, index and CC state stay on the device.  Continuous
+ * batching: one persistent launch runs while requests arrive; each arrived
+ * batch is copied from pinned host memory, prepared on a second stream and
+ * admitted into the running launch, whose groups take its transactions as
+ * they free up, beside those still running.  A launch is closed and a new one
+ * opened when the next batch would exceed the admission limits (transactions
+ * per launch; MVCC version pool, bounded by free device memory) or before an
+ * idle gap longer than a quarter of the watchdog period; the watchdog bounds
+ * how long a launch waits for an admission.  tx_latency_ms (may be null)
+ * receives, per transaction, its commit time minus the arrival of its batch
+ * (commits are stamped with the device wall clock, mapped to host time
+ * through a clock sample).  Order keys from gcctb_fetch are {launch, key0,
+ * key1}.  Not available for gputx (it ranks its whole input before running). */
 typedef struct gcctb_serve_config {
   uint64_t max_launch_txs;  /* admission: transactions per launch (0 = no limit) */
   uint64_t pool_cap_bytes;  /* admission, MVCC: version-pool budget, 24 bytes per write
@@ -158,13 +169,15 @@ typedef struct gcctb_serve_config {
 typedef struct gcctb_serve_stats {
   uint64_t txs, committed, aborts;
   uint64_t launches;
-  uint64_t deferred;        /* times an arrived batch waited for a later launch */
+  uint64_t deferred;        /* times an arrived batch waited for a running launch to drain */
   uint64_t error_flags;
   double tot_ms;            /* call start to the last completion */
-  double busy_ms;           /* sum of launch durations (copy, prepare, run, restore) */
+  double busy_ms;           /* sum of launch durations (open to drained and restored) */
   double txps;              /* committed / tot_ms */
   double p50_ms, p99_ms, max_ms, mean_ms;  /* commit latency */
   double mean_launch_txs;
+  uint64_t admitted_live;   /* batches admitted into a launch that was already running */
+  uint64_t pool_nodes;      /* MVCC: versions one launch may hold (admission limit) */
 } gcctb_serve_stats;
 int gcctb_serve(gcctb_ctx* ctx, const gcctb_serve_config* cfg, const uint64_t* batch_sizes,
                 const double* arrival_ms, uint64_t nbatches, double* tx_latency_ms, gcctb_serve_stats* out);
@@ -195,6 +208,34 @@ int gcctb_fetch_cc(gcctb_ctx* ctx, uint64_t* words);
  * it was before the run) and counts reads and final cells that disagree with
  * the GPU (*bad) and transactions that did not commit (*uncommitted). */
 int gcctb_check(gcctb_ctx* ctx, const uint64_t* init_table, uint64_t* bad, uint64_t* uncommitted);
+
+/* Event log and conflict-graph check (the debug mode's ExecInfo::Check,
+ * include/runtime.cuh:39-81, src/runtime.cu:23-158).  An event is a read
+ * (type 0) or write (type 1) of object oid (a row) by transaction tid, or its
+ * commit (type 2); ids order the events in time (AddEvent draws them from one
+ * global counter).  gcctb_conflict_check runs on the host alone: a
+ * transaction is active on an object from its first access to its commit; a
+ * read links the reader to every other active writer of the object, a write
+ * to every other active accessor; transactions without a commit are counted
+ * (and listed in the report as UNCOMMITTED), and a depth-first search reports
+ * the first cycle it closes (*cycle_len transactions, the first cycle_cap of
+ * them in cycle).  report (may be null) receives the text the reference's
+ * check prints: SCAN OK, UNCOMMITTED lines, for a cycle the BAD TX lines and
+ * its events, then GOOD or BAD. */
+typedef struct gcctb_event {
+  uint64_t id;
+  uint64_t oid;
+  int32_t tid;
+  int32_t type;
+} gcctb_event;
+int gcctb_conflict_check(const gcctb_event* events, uint64_t n, uint64_t ntx, uint64_t* uncommitted, uint64_t* cycle,
+                         uint64_t cycle_cap, uint64_t* cycle_len, char* report, uint64_t report_len);
+/* The event log of the last plain or graph run of a context created with
+ * debug_events = 1: every op of each transaction's committed attempt, stamped
+ * when the access took effect, then the transaction's commit.  events must
+ * hold nops + ntx entries; *n receives the count (GCCTB_ESTATE without a
+ * logged run). */
+int gcctb_fetch_events(gcctb_ctx* ctx, gcctb_event* events, uint64_t cap, uint64_t* n);
 
 /* Workload generation (host, deterministic in (seed, tx, draw); counter-based,
  * multi-threaded).  Zipf draw follows generation/gen_ycsb.cu:76-102 with
@@ -250,7 +291,13 @@ int gcctb_sort_pairs(void* keys, uint32_t* vals, uint64_t n, int32_t key_bytes, 
  *         7: pairs of random coherent 8-byte loads on two lines
  *         8: load a word, load a cell in its 128-byte line, CAS the word
  *         9: the same with the cell in another line
- *            (rate in pairs/s for 6 and 7, reads/s for 8 and 9; n_words >= 16)
+ *        10: atomic add to 16 words that share one 128-byte line
+ *        11: atomic add to 16 words on 16 lines
+ *        12: atomic add to one word (one lane per wave) while the other lanes
+ *            poll words of the same line with coherent loads
+ *        13: the same with the polled words on other lines
+ *            (rate in pairs/s for 6 and 7, reads/s for 8 and 9, adds to the
+ *            one word for 12 and 13; n_words >= 16, >= 256 for 10..13)
  * threads x iters operations; *rate in operations/s, *ms the timed launch. */
 int gcctb_microbench(int32_t which, uint64_t n_words, uint64_t threads, uint64_t iters, double* rate, double* ms);
 

@@ -49,6 +49,15 @@ int64_t orc_replay(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table,
  * and tie count. */
 int64_t orc_replay_mt(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
                       const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads, int threads);
+/* Whole-row mode (every op reads or writes all cols cells of its row, like
+ * the reference's Item copies, benchmark/ycsb.cu:57,86-91): cell c written by
+ * op j of transaction t holds orc_write_cell(seed, t, j, c) = mix64(write
+ * value + c); a read returns orc_row_digest of the row it saw.
+ * orc_replay_rows is orc_replay_mt in that mode. */
+uint64_t orc_write_cell(uint64_t seed, uint64_t tx, uint32_t op, uint32_t col);
+uint64_t orc_row_digest(const uint64_t* row, uint32_t cols);
+int64_t orc_replay_rows(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                        const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads, int threads);
 /* Replay in plain transaction-id order (GaccO's serial order inside a batch). */
 void orc_replay_txid(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
                      const uint64_t* tx_off, uint64_t ntx, uint64_t* reads);

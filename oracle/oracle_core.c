@@ -30,6 +30,18 @@ uint64_t orc_cell_init(uint64_t seed, uint64_t row, uint32_t col, uint32_t cols)
 uint64_t orc_write_value(uint64_t seed, uint64_t tx, uint32_t op) {
   return orc_hash3(seed ^ TAG_WRITE, tx, op);
 }
+/* Whole-row mode (benchmark/ycsb.cu:57,86-91 copies the 104-byte Item): cell
+ * c of a written row holds mix64(write value + c); a read returns the row's
+ * cells folded from DIGEST_INIT ("ROW") by h = mix64(h ^ cell). */
+#define ORC_DIGEST_INIT 0x524f570000000000ull
+uint64_t orc_write_cell(uint64_t seed, uint64_t tx, uint32_t op, uint32_t col) {
+  return mix64(orc_write_value(seed, tx, op) + col);
+}
+uint64_t orc_row_digest(const uint64_t* row, uint32_t cols) {
+  uint64_t h = ORC_DIGEST_INIT;
+  for (uint32_t c = 0; c < cols; ++c) h = mix64(h ^ row[c]);
+  return h;
+}
 void orc_seed_table(uint64_t seed, uint64_t rows, uint32_t cols, uint64_t* cells) {
   for (uint64_t r = 0; r < rows; ++r)
     for (uint32_t c = 0; c < cols; ++c) cells[r * cols + c] = orc_cell_init(seed, r, c, cols);
@@ -190,6 +202,7 @@ void orc_replay_txid(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* tabl
 typedef struct {
   uint64_t rows, row_lo, row_hi, ntx, seed;
   uint32_t cols;
+  int row_ops;
   uint64_t* table;
   const uint64_t* ops;
   const uint64_t* tx_off;
@@ -209,6 +222,17 @@ static void* replay_part(void* arg) {
         if (p->reads) p->reads[i] = 0;
         continue;
       }
+      if (p->row_ops) {
+        /* whole-row mode: a write stores every cell, a read returns the digest */
+        uint64_t* r = p->table + row * p->cols;
+        if ((op >> 40) & 1) {
+          for (uint32_t c = 0; c < p->cols; ++c) r[c] = orc_write_cell(p->seed, t, (uint32_t)(i - p->tx_off[t]), c);
+          if (p->reads) p->reads[i] = 0;
+        } else if (p->reads) {
+          p->reads[i] = orc_row_digest(r, p->cols);
+        }
+        continue;
+      }
       uint64_t* cell = p->table + row * p->cols + col;
       if ((op >> 40) & 1) {
         *cell = orc_write_value(p->seed, t, (uint32_t)(i - p->tx_off[t]));
@@ -221,8 +245,23 @@ static void* replay_part(void* arg) {
   return NULL;
 }
 
+static int64_t replay_parts(uint64_t rows, uint32_t cols, uint64_t seed, int row_ops, uint64_t* table,
+                            const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
+                            uint64_t* reads, int threads);
+
 int64_t orc_replay_mt(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
                       const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads, int threads) {
+  return replay_parts(rows, cols, seed, 0, table, ops, tx_off, ntx, order, reads, threads);
+}
+
+int64_t orc_replay_rows(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* table, const uint64_t* ops,
+                        const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* reads, int threads) {
+  return replay_parts(rows, cols, seed, 1, table, ops, tx_off, ntx, order, reads, threads);
+}
+
+static int64_t replay_parts(uint64_t rows, uint32_t cols, uint64_t seed, int row_ops, uint64_t* table,
+                            const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
+                            uint64_t* reads, int threads) {
   if (threads < 1) threads = 1;
   if (threads > 256) threads = 256;
   okey_t* ord = (okey_t*)malloc(sizeof(okey_t) * (ntx ? ntx : 1));
@@ -249,6 +288,7 @@ int64_t orc_replay_mt(uint64_t rows, uint32_t cols, uint64_t seed, uint64_t* tab
     part[k].ntx = ntx;
     part[k].seed = seed;
     part[k].cols = cols;
+    part[k].row_ops = row_ops;
     part[k].table = table;
     part[k].ops = ops;
     part[k].tx_off = tx_off;

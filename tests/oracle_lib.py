@@ -50,6 +50,13 @@ def lib() -> C.CDLL:
         L.orc_replay_mt.restype = C.c_int64
         L.orc_replay_mt.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64, _u64p,
                                     _u64p, C.c_int]
+        L.orc_replay_rows.restype = C.c_int64
+        L.orc_replay_rows.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64, _u64p,
+                                      _u64p, C.c_int]
+        L.orc_write_cell.restype = C.c_uint64
+        L.orc_write_cell.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_row_digest.restype = C.c_uint64
+        L.orc_row_digest.argtypes = [_u64p, C.c_uint32]
         L.orc_replay_txid.restype = None
         L.orc_replay_txid.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64,
                                       _u64p]
@@ -88,6 +95,22 @@ def replay(rows, cols, seed, table, ops, tx_off, order, threads=1, in_place=Fals
         ties = lib().orc_replay(rows, cols, seed, t, _a64(ops), _a64(tx_off), ntx, _a64(order), reads)
     assert ties >= 0, "oracle allocation failed"
     return t, reads[: len(ops)], ties
+
+
+def replay_rows(rows, cols, seed, table, ops, tx_off, order, threads=1):
+    """Serial replay in whole-row mode (every op moves its whole row; reads
+    return row digests) -> (final table, reads, ties)."""
+    t = np.array(table, dtype=np.uint64, copy=True)
+    reads = np.zeros(max(1, len(ops)), dtype=np.uint64)
+    ties = lib().orc_replay_rows(rows, cols, seed, t, _a64(ops), _a64(tx_off), len(tx_off) - 1, _a64(order), reads,
+                                 threads)
+    assert ties >= 0, "oracle allocation failed"
+    return t, reads[: len(ops)], ties
+
+
+def row_digest(row) -> int:
+    r = _a64(row)
+    return lib().orc_row_digest(r, len(row))
 
 
 def replay_txid(rows, cols, seed, table, ops, tx_off):

@@ -102,3 +102,59 @@ def test_check_ops_rejects_repeated_keys():
     # the same key in two different transactions is fine
     two = g.make_ops([5, 5], [1, 2], [True, False])
     assert g.check_ops(g.Ops(two, np.array([0, 1, 2], dtype=np.uint64))) == 1
+
+
+def _events(rows):
+    return np.array([(i + 1, oid, tid, typ) for i, (tid, typ, oid) in enumerate(rows)], dtype=g.EVENT_DTYPE)
+
+
+def test_conflict_check_serial_history_is_good():
+    # T0 writes x then commits; T1 reads x and writes y; T2 reads y; commits in order
+    ev = _events([(0, 1, 5), (0, 2, 0), (1, 0, 5), (1, 1, 7), (1, 2, 0), (2, 0, 7), (2, 2, 0)])
+    r = g.conflict_check(ev, 3)
+    assert r["cycle"] == [] and r["uncommitted"] == 0
+    assert r["report"].startswith("SCAN OK\n") and r["report"].endswith("GOOD\n")
+
+
+def test_conflict_check_reports_injected_cycle():
+    """A non-serializable interleaving (src/runtime.cu:23-127): T0 and T1 both
+    write x and y while the other is still active, in opposite orders, so
+    T1 -> T0 on x and T0 -> T1 on y; the check prints BAD with the cycle's
+    transactions and their events."""
+    ev = _events([(0, 1, 10),   # T0 writes x
+                  (1, 1, 20),   # T1 writes y
+                  (1, 1, 10),   # T1 writes x while T0 is active: T1 -> T0
+                  (0, 1, 20),   # T0 writes y while T1 is active: T0 -> T1
+                  (0, 2, 0), (1, 2, 0),
+                  (2, 0, 30), (2, 2, 0)])
+    r = g.conflict_check(ev, 3)
+    assert sorted(r["cycle"]) == [0, 1]
+    rep = r["report"]
+    assert "!!!!!!!!!" in rep and "BAD TX 0" in rep and "BAD TX 1" in rep and rep.endswith("BAD\n")
+    assert "BAD ST" in rep and "tid 0 oid10 w" in rep
+
+
+def test_conflict_check_lists_uncommitted_and_read_edges():
+    # T1 reads x while T0 (active) wrote it: T1 -> T0 only (no cycle); T2 never commits
+    ev = _events([(0, 1, 4), (1, 0, 4), (0, 2, 0), (1, 2, 0), (2, 0, 9)])
+    r = g.conflict_check(ev, 3)
+    assert r["uncommitted"] == 1 and "UNCOMMITTED 2\n" in r["report"] and r["cycle"] == []
+    # read-read never conflicts; a read after an active writer does, in a cycle with a write-after-read
+    ev = _events([(0, 0, 1), (1, 0, 1), (0, 2, 0), (1, 2, 0)])
+    assert g.conflict_check(ev, 2)["cycle"] == []
+    ev = _events([(0, 1, 1), (1, 0, 1), (1, 0, 2), (0, 1, 2), (0, 2, 0), (1, 2, 0)])
+    assert sorted(g.conflict_check(ev, 2)["cycle"]) == [0, 1]
+
+
+def test_conflict_check_long_chain_no_recursion():
+    # 200,000 transactions in a chain of read-after-write edges: the search is iterative
+    n = 200000
+    rows = []
+    for t in range(n):
+        rows.append((t, 1, t))          # T_t writes object t
+        if t:
+            rows.append((t, 0, t - 1))  # reads object t-1 while T_{t-1} is active
+    for t in range(n):
+        rows.append((t, 2, 0))
+    r = g.conflict_check(_events(rows), n, report_len=4096)
+    assert r["cycle"] == [] and r["uncommitted"] == 0

@@ -156,7 +156,64 @@ def test_cli_debug_run(scheme, tmp_path):
     assert p.returncode == 0, p.stderr
     assert f"TRY {scheme},3,8,2048" in p.stdout
     assert "SCAN OK" in p.stdout and "GOOD" in p.stdout and "BAD" not in p.stdout
+    assert "REPLAY OK" in p.stdout
     line = (tmp_path / (dataset + ".txt")).read_text().strip().splitlines()[-1]
     fields = line.split(",")
     assert len(fields) == 15 and fields[0] == scheme
     assert float(fields[4]) > 0  # throughput
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["tictoc", "mvcc"])
+def test_cli_reference_txs_file(scheme, tmp_path):
+    """A reference-format .txs file (136-byte YCSBTx records with request_cnt
+    below 16, as the reference's generator leaves after dropping repeated
+    keys) through the drop-in driver: whole-row ops on 104-byte rows, the
+    debug check GOOD, and the CSV line."""
+    rows, ntx = 4096, 3000
+    w = g.generate(rows, ntx, theta=0.7, read_ratio=0.8, len_min=3, len_max=15, cols=13)
+    assert int(np.diff(w.tx_off.astype(np.int64)).max()) < 16
+    path = str(tmp_path / "ref.txs")
+    g.write_txs(path, w)
+    assert os.path.getsize(path) == 136 * ntx
+    args = [CLI, path, str(rows), str(ntx), scheme, "3", "8", "1024", "1"]
+    print(f"[cli-txs] {' '.join(args[1:])}", flush=True)
+    p = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "SCAN OK" in p.stdout and "GOOD" in p.stdout and "BAD" not in p.stdout
+    fields = open(path + ".txt").read().strip().splitlines()[-1].split(",")
+    assert len(fields) == 15 and fields[0] == scheme and float(fields[4]) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["plain", "graph"])
+@pytest.mark.parametrize("scheme", list(g.SCHEMES))
+def test_event_log_conflict_check(scheme, form):
+    """The debug event log (AddEvent, include/runtime.cuh:64-81): every op of
+    every committed attempt and every commit is stamped; the reference's
+    conflict-graph check over it (src/runtime.cu:23-158) finds no cycle and
+    no uncommitted transaction, and the run still equals its serial replay."""
+    rows, ntx = 512, 1500
+    w = g.generate(rows, ntx, theta=0.8, read_ratio=0.6, len_min=4, len_max=12)
+    print(f"[events] scheme={scheme} form={form}", flush=True)
+    with g.Context(scheme, rows=rows, warp_density=3, block_size=8, batch_size=500, debug_events=True) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run(form)
+        res = ctx.fetch()
+        ev = ctx.fetch_events()
+    assert st["committed"] == ntx
+    assert ev.size == w.nops + ntx
+    assert np.unique(ev["id"]).size == ev.size and ev["id"].min() >= 1
+    # a commit is stamped after every op of its transaction
+    commit = np.zeros(ntx, dtype=np.uint64)
+    c = ev[ev["type"] == 2]
+    commit[c["tid"]] = c["id"]
+    ops = ev[ev["type"] != 2]
+    assert np.all(ops["id"] < commit[ops["tid"]])
+    r = g.conflict_check(ev, ntx)
+    assert r["uncommitted"] == 0 and r["cycle"] == [], r["report"][-2000:]
+    assert r["report"].endswith("GOOD\n")
+    init = orc.seed_table(g.SEED, rows, 10)
+    exp_t, exp_r, _ = orc.replay(rows, 10, g.SEED, init, w.ops, w.tx_off, res["order"])
+    assert np.array_equal(res["table"], exp_t)

@@ -97,8 +97,8 @@ def test_uneven_batch_stream(scheme):
         assert np.array_equal(results["plain"], results["persistent"])
 
 
-ADAPTIVE_OFF = ("the adaptive window is disabled: with Silo it faulted the GPU (out-of-bounds access) "
-                "in a serving run, cause not found")
+ADAPTIVE_OFF = ("the adaptive window is not built: with Silo it faulted the GPU (out-of-bounds access) "
+                "in a serving run, the cause was not found and its code was taken out of the kernels")
 
 
 @pytest.mark.gpu

@@ -81,3 +81,52 @@ def test_replay_parity(scheme, case):
             assert not np.any((cc >> np.uint64(1)) & np.uint64(0x7FFFFFFF)), "a lock is still held"
         else:
             assert not np.any(cc), "CC words are reset after every batch"
+
+
+# Whole-row mode (the reference's 104-byte Item copies, benchmark/ycsb.cu:57,86-91):
+# rows of 13 u64 cells, every op moves its whole row, reads return row digests.
+ROW_CASES = {
+    # name: (rows, ntx, len_min, len_max, theta, read_ratio, batch, wd, bs)
+    "item": (2048, 1500, 4, 16, 0.8, 0.5, 500, 4, 8),
+    "hot": (64, 600, 8, 8, 0.99, 0.5, 200, 3, 8),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["plain", "graph"])
+@pytest.mark.parametrize("case", list(ROW_CASES))
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_row_mode_replay(scheme, case, form):
+    rows, ntx, lmin, lmax, theta, rr, batch, wd, bs = ROW_CASES[case]
+    cols = 13
+    print(f"[row-mode] scheme={scheme} case={case} form={form}", flush=True)
+    w = g.generate(rows, ntx, theta=theta, read_ratio=rr, len_min=lmin, len_max=lmax, cols=cols)
+    with g.Context(scheme, rows=rows, cols=cols, warp_density=wd, block_size=bs, batch_size=batch,
+                   row_ops=True) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(w)
+        st = ctx.run(form)
+        res = ctx.fetch()
+        ranks = ctx.fetch_ranks() if scheme == "gputx" else None
+    assert st["committed"] == ntx and st["error_flags"] == 0
+    init = orc.seed_table(g.SEED, rows, cols)
+    exp_t, exp_r, ties = orc.replay_rows(rows, cols, g.SEED, init, w.ops, w.tx_off, res["order"])
+    assert ties == 0
+    isr = ~g.op_is_write(w.ops)
+    assert np.array_equal(res["reads"][isr], exp_r[isr]), "row digests differ from the whole-row replay"
+    assert np.array_equal(res["table"], exp_t), "table differs from the whole-row replay"
+    if scheme == "gacco":
+        assert np.array_equal(res["order"][1::3], np.arange(ntx, dtype=np.uint64))
+    if scheme == "gputx":
+        rank_o, _ = orc.gputx_rank(rows, w.ops, w.tx_off)
+        assert np.array_equal(ranks[0], rank_o)
+
+
+@pytest.mark.gpu
+def test_row_mode_refuses_persistent_form():
+    with g.Context("silo", rows=64, cols=13, row_ops=True) as ctx:
+        ctx.seed_table()
+        ctx.load_ops(g.generate(64, 10, theta=0.5, read_ratio=0.5, len_min=4, cols=13))
+        with pytest.raises(g.GcctbError) as e:
+            ctx.run("persistent")
+        assert e.value.code == -1

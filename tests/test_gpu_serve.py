@@ -51,10 +51,10 @@ def test_serve_replay(scheme):
 @pytest.mark.gpu
 @pytest.mark.parametrize("scheme", ["mvcc", "silo", "gacco"])
 def test_serve_mix_vs_alone(scheme):
-    """The same requests mixed into large launches and run one batch per
-    launch: both histories replay exactly; where the outcome is fixed by the
-    input it is bit-identical (gacco: transaction-id order in both; a
-    conflict-free stream: every scheme)."""
+    """The same requests mixed into one continuously fed launch and run one
+    batch per launch: both histories replay exactly; where the outcome is
+    fixed by the input it is bit-identical (gacco: transaction-id order in
+    both; a conflict-free stream: every scheme)."""
     s = small_stream(0.9, seed=g.SEED + 1)
     print(f"[mix] scheme={scheme} ntx={s.ntx}", flush=True)
     st_mix, mix = serve(scheme, s)
@@ -76,6 +76,41 @@ def test_serve_mix_vs_alone(scheme):
     _, a = serve(scheme, cf)
     _, b = serve(scheme, cf, max_launch_txs=1)
     assert np.array_equal(a["table"], b["table"]) and np.array_equal(a["reads"], b["reads"])
+
+
+def hot_then_cold(hot_txs=20000, cold_batches=5, cold_txs=100, length=8, seed=g.SEED + 3):
+    """Batch 0: hot_txs transactions on the first 64 rows (theta 0.99, read
+    ratio 0.5: a long, contended batch); then cold batches of transactions
+    on rows that no other transaction touches."""
+    hot = g.generate(64, hot_txs, theta=0.99, read_ratio=0.5, len_min=length, len_max=length, seed=seed)
+    n = cold_batches * cold_txs
+    rng = np.random.default_rng(seed)
+    keys = (64 + rng.permutation(ROWS - 64)[: n * length]).astype(np.uint64)
+    keys = np.sort(keys.reshape(n, length), axis=1).reshape(-1)
+    cols = rng.integers(0, 10, n * length).astype(np.uint64)
+    cold = g.Ops(g.make_ops(keys, cols, rng.random(n * length) < 0.3),
+                 np.arange(0, n * length + 1, length, dtype=np.uint64))
+    return wl.Stream(wl.concat([hot, cold]), [hot_txs] + [cold_txs] * cold_batches)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["silo", "mvcc", "tpl_nw", "tictoc", "gacco"])
+def test_serve_continuous(scheme):
+    """Continuous batching: batches that arrive while the launch runs join
+    it (one launch serves every batch), the late batches commit while the
+    first batch's contended transactions are still running, and the whole
+    history equals its serial replay."""
+    s = hot_then_cold()
+    arr = np.array([0.0, 2.0, 4.0, 6.0, 8.0, 10.0])
+    print(f"[continuous] scheme={scheme} ntx={s.ntx} batches={len(s.batch_sizes)}", flush=True)
+    st, res = serve(scheme, s, arr)
+    assert st["committed"] == s.ntx and st["error_flags"] == 0
+    assert st["launches"] == 1 and st["admitted_live"] == len(s.batch_sizes) - 1 and st["deferred"] == 0
+    batch_of = np.repeat(np.arange(len(s.batch_sizes)), s.batch_sizes)
+    commit_ms = st["latency_ms"] + arr[batch_of]
+    hot = s.batch_sizes[0]
+    assert commit_ms[hot:].min() < commit_ms[:hot].max(), "late batches should commit beside the running batch"
+    check_replay(s.ops, res, ROWS)
 
 
 @pytest.mark.gpu
@@ -113,8 +148,9 @@ def test_serve_poisson_latency():
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="the adaptive window is disabled: this test faulted the GPU with Silo (out-of-bounds "
-                         "access) and once returned reads that differ from the serial replay; cause not found")
+@pytest.mark.skip(reason="the adaptive window is not built: this test faulted the GPU with Silo (out-of-bounds "
+                         "access) and once returned reads that differ from the serial replay; the cause was not "
+                         "found and the window's code was taken out of the kernels")
 @pytest.mark.parametrize("scheme", ["mvcc", "silo"])
 def test_serve_adaptive(scheme):
     """Serving with the adaptive admission window in every launch."""

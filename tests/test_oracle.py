@@ -321,3 +321,66 @@ def test_cpu_path_matches_serial_replay(scheme, case):
     if scheme == "gacco":
         t2, r2 = orc.replay_txid(rows, 10, g.SEED, init, w.ops, w.tx_off)
         assert np.array_equal(table, t2)
+
+
+GOLDEN = 0x9E3779B97F4A7C15
+M64 = (1 << 64) - 1
+
+
+def _mix64(z):
+    # the splitmix64 finalizer, through the pinned splitmix64: mix64(z) = splitmix64(z - golden)
+    return orc.lib().orc_splitmix64((z - GOLDEN) & M64)
+
+
+def test_row_digest_and_write_cell_forms():
+    """Whole-row mode's digest folds the cells with the splitmix64 finalizer
+    from "ROW", and a written cell is the finalizer of (write value + column)."""
+    init = 0x524F570000000000
+    a, b = 0x0123456789ABCDEF, 0xFEDCBA9876543210
+    assert orc.row_digest([a]) == _mix64(init ^ a)
+    assert orc.row_digest([a, b]) == _mix64(_mix64(init ^ a) ^ b)
+    assert orc.row_digest([b, a]) != orc.row_digest([a, b])
+    for t, j, c in [(0, 0, 0), (5, 3, 12), (123456, 31, 7)]:
+        w = orc.lib().orc_write_value(g.SEED, t, j)
+        assert orc.lib().orc_write_cell(g.SEED, t, j, c) == _mix64((w + c) & M64)
+
+
+def test_replay_rows_one_column_matches_column_replay():
+    """With one column, whole-row replay is column replay seen through the
+    finalizer: a written cell is mix64 of the column-mode value, and a read's
+    digest is mix64(ROW ^ the row's cell at the time)."""
+    rows, ntx = 64, 400
+    w = g.generate(rows, ntx, theta=0.9, read_ratio=0.6, len_min=1, len_max=8, cols=1, seed=11)
+    order = np.zeros(3 * ntx, dtype=np.uint64)
+    order[1::3] = np.random.default_rng(3).permutation(ntx)
+    init = orc.seed_table(g.SEED, rows, 1)
+    tc, rc, _ = orc.replay(rows, 1, g.SEED, init, w.ops, w.tx_off, order)
+    tr, rr, ties = orc.replay_rows(rows, 1, g.SEED, init, w.ops, w.tx_off, order)
+    assert ties == 0
+    isw = g.op_is_write(w.ops)
+    written = np.zeros(rows, dtype=bool)
+    written[g.op_key(w.ops)[isw].astype(np.int64)] = True
+    for r in range(rows):
+        assert int(tr[r]) == (_mix64(int(tc[r])) if written[r] else int(init[r]))
+    keys = g.op_key(w.ops).astype(np.int64)
+    for i in np.nonzero(~isw)[0]:
+        v = int(rc[i])
+        cell = v if v == int(init[keys[i]]) else _mix64(v)
+        assert int(rr[i]) == _mix64(0x524F570000000000 ^ cell)
+
+
+@pytest.mark.parametrize("threads", [2, 5])
+def test_replay_rows_mt_matches_serial(threads):
+    rows, ntx, cols = 300, 500, 13
+    w = g.generate(rows, ntx, theta=0.8, read_ratio=0.5, len_min=4, len_max=16, cols=cols, seed=12)
+    order = np.zeros(3 * ntx, dtype=np.uint64)
+    order[1::3] = np.random.default_rng(4).permutation(ntx)
+    init = orc.seed_table(g.SEED, rows, cols)
+    t1, r1, _ = orc.replay_rows(rows, cols, g.SEED, init, w.ops, w.tx_off, order, threads=1)
+    t2, r2, _ = orc.replay_rows(rows, cols, g.SEED, init, w.ops, w.tx_off, order, threads=threads)
+    assert np.array_equal(t1, t2) and np.array_equal(r1, r2)
+    # every written row holds one transaction's write_cell values in all of its cells
+    isw = g.op_is_write(w.ops)
+    t1 = t1.reshape(rows, cols)
+    for r in set(g.op_key(w.ops)[isw].astype(np.int64).tolist()):
+        assert int(t1[r, 1]) != int(t1[r, 0])
