@@ -128,6 +128,13 @@ def load_library() -> C.CDLL:
         "gcctb_read_txs_file": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p,
                                           C.POINTER(C.c_uint64)]),
         "gcctb_write_txs_file": (C.c_int, [C.c_char_p, _u64p, _u64p, C.c_uint64]),
+        "gcctb_cpu_seed_table": (C.c_int, [_u64p, C.c_uint64, C.c_uint32, C.c_uint64]),
+        "gcctb_cpu_run": (C.c_int, [C.POINTER(Config), C.c_int32, _u64p, _u64p, _u64p, C.c_uint64, _u64p, _u64p,
+                                    C.POINTER(Stats)]),
+        "gcctb_check_host": (C.c_int, [C.POINTER(Config), _u64p, _u64p, _u64p, _u64p, C.c_uint64, _u64p, _u64p,
+                                       C.POINTER(C.c_uint64)]),
+        "gcctb_format_result": (C.c_int, [C.POINTER(Stats), C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                          C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint64]),
         "gcctb_device_info": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
         "gcctb_check": (C.c_int, [P, _u64p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "gcctb_microbench": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
@@ -214,6 +221,45 @@ def read_txs(path: str, ntx: int, cols: int = 10, seed: int = SEED) -> Ops:
 def write_txs(path: str, w: Ops) -> None:
     _check(load_library().gcctb_write_txs_file(path.encode(), np.ascontiguousarray(w.ops),
                                                np.ascontiguousarray(w.tx_off), w.ntx))
+
+
+def cpu_run(scheme: str, rows: int, w: "Ops", *, cols: int = 10, batch_size: int = 65536, threads: int = 0,
+            row_ops: bool = False, seed: int = SEED, table: np.ndarray = None) -> tuple:
+    """The CPU path (gcctb_cpu_run) on a seeded table, or on `table` (updated
+    in place).  Returns (stats dict, {"table", "reads", "order"})."""
+    lib = load_library()
+    if table is None:
+        table = np.zeros(rows * cols, dtype=np.uint64)
+        _check(lib.gcctb_cpu_seed_table(table, rows, cols, seed))
+    cfg = Config(cc_scheme=scheme.encode(), batch_size=batch_size, rows=rows, cols=cols, seed=seed,
+                 row_ops=1 if row_ops else 0)
+    reads = np.zeros(max(1, w.nops), dtype=np.uint64)
+    order = np.zeros(max(1, 3 * w.ntx), dtype=np.uint64)
+    st = Stats()
+    _check(lib.gcctb_cpu_run(C.byref(cfg), threads, table, np.ascontiguousarray(w.ops), np.ascontiguousarray(w.tx_off),
+                             w.ntx, reads, order, C.byref(st)))
+    return st.as_dict(), {"table": table, "reads": reads[: w.nops], "order": order[: 3 * w.ntx]}
+
+
+def check_host(rows: int, cols: int, w: "Ops", init: np.ndarray, res: dict, *, row_ops: bool = False,
+               seed: int = SEED) -> int:
+    """gcctb_check_host: reads and cells that differ from the serial replay."""
+    cfg = Config(cc_scheme=b"tpl_nw", batch_size=1, rows=rows, cols=cols, seed=seed, row_ops=1 if row_ops else 0)
+    bad = C.c_uint64(0)
+    _check(load_library().gcctb_check_host(C.byref(cfg), np.ascontiguousarray(init), res["table"],
+                                           np.ascontiguousarray(w.ops), np.ascontiguousarray(w.tx_off), w.ntx,
+                                           res["order"], np.ascontiguousarray(res["reads"]), C.byref(bad)))
+    return bad.value
+
+
+def format_result(stats: dict, cc_type: str, warp_density: int, block_size: int, batch_size: int,
+                  txcnt: int) -> tuple:
+    """The driver's result block (ExecInfo::Analyse): (CSV line, stdout text)."""
+    st = Stats(**{k: stats[k] for k, _ in Stats._fields_ if k in stats})
+    csv, text = C.create_string_buffer(4096), C.create_string_buffer(8192)
+    _check(load_library().gcctb_format_result(C.byref(st), cc_type.encode(), warp_density, block_size, batch_size,
+                                              txcnt, csv, len(csv), text, len(text)))
+    return csv.value.decode(), text.value.decode()
 
 
 def op_key(ops: np.ndarray) -> np.ndarray:

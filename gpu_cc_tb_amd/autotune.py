@@ -14,6 +14,11 @@ of mixed skew), each entry {"warp_density", "block_size", "window",
 "adaptive", "txps"}.  `lookup` maps a workload's skew and read ratio to the
 nearest measured bucket.
 
+The plain per-batch form is tuned too, with the batch size as a fourth
+parameter (the reference's batch_size argument, benchmark_cpu/ycsb.cu:96):
+entries "plain-t<theta>-r<read>" hold {"warp_density", "block_size",
+"batch_size", "txps"}, and `lookup_plain` reads them.
+
     python -m gpu_cc_tb_amd.autotune --out gpu_cc_tb_amd/tuned.json
 """
 from __future__ import annotations
@@ -35,6 +40,10 @@ READS = {"w": 0.5, "r": 0.9, "o": 1.0}
 
 WD = [1, 2, 3, 4]
 BS = [4, 8]
+# plain form: lanes per transaction, block size (in 32-lane units) and batch size
+PLAIN_WD = [1, 2, 3, 4, 5]
+PLAIN_BS = [4, 8, 16]
+PLAIN_BATCHES = [4096, 16384, 65536, 262144]
 # admission windows (transactions in flight; 0 = device-filling).  The
 # adaptive window (-1) is left out: the library refuses it in this build.
 WINDOWS = [0, 32768, 16384, 8192, 4096, 1024, 512, 256]
@@ -83,6 +92,57 @@ def lookup(scheme: str, theta: float, read_ratio: float, table: dict | None = No
                 "adaptive": bool(e.get("adaptive", False))}
     wd, bs = DEFAULT_SHAPE[scheme]
     return {"warp_density": wd, "block_size": bs, "window": 0, "adaptive": False}
+
+
+def lookup_plain(scheme: str, theta: float, read_ratio: float, table: dict | None = None) -> dict:
+    """Tuned {warp_density, block_size, batch_size} of the plain form for a
+    workload: the nearest measured skew with the same read bucket (else any
+    plain entry), or the default shape at batch 65,536."""
+    table = load_table() if table is None else table
+    ent = {k[len("plain-"):]: v for k, v in table.get(scheme, {}).items() if k.startswith("plain-t")}
+    key = bucket(theta, read_ratio)
+    if key not in ent and ent:
+        same_r = [k for k in ent if k.endswith("-r" + read_bucket(read_ratio))] or list(ent)
+        key = min(same_r, key=lambda k: abs(THETAS[k[1]] - theta))
+    if key in ent:
+        e = ent[key]
+        return {"warp_density": e["warp_density"], "block_size": e["block_size"], "batch_size": e["batch_size"]}
+    wd, bs = DEFAULT_SHAPE[scheme]
+    return {"warp_density": wd, "block_size": bs, "batch_size": 65536}
+
+
+def measure_plain(scheme: str, rows: int, ops, wd: int, bs: int, batch: int) -> float:
+    """Committed transactions per second of one plain run (0 on a failed run)."""
+    with Context(scheme, rows=rows, warp_density=wd, block_size=bs, batch_size=batch) as c:
+        c.seed_table()
+        c.load_ops(ops)
+        st = c.run("plain", check=False)
+    if st["rc"] != 0 or st["committed"] != ops.ntx:
+        return 0.0
+    return ops.ntx / (st["tot_ms"] / 1e3)
+
+
+def tune_plain(scheme: str, rows: int, ops, log=print) -> dict:
+    """Coordinate descent over batch size, warp_density, block_size, then the
+    batch size again, for the plain form."""
+    wd, bs = DEFAULT_SHAPE[scheme]
+    batch = 65536
+    cache: dict[tuple, float] = {}
+
+    def score(w, b, n):
+        k = (w, b, n)
+        if k not in cache:
+            t0 = time.time()
+            cache[k] = measure_plain(scheme, rows, ops, w, b, n)
+            log(json.dumps({"scheme": scheme, "form": "plain", "wd": w, "bs": b, "batch": n,
+                            "txps": round(cache[k]), "s": round(time.time() - t0, 2)}))
+        return cache[k]
+
+    batch = max(PLAIN_BATCHES, key=lambda n: score(wd, bs, n))
+    wd = max(PLAIN_WD, key=lambda w: score(w, bs, batch))
+    bs = max(PLAIN_BS, key=lambda b: score(wd, b, batch))
+    batch = max(PLAIN_BATCHES, key=lambda n: score(wd, bs, n))
+    return {"warp_density": wd, "block_size": bs, "batch_size": batch, "txps": round(score(wd, bs, batch))}
 
 
 def measure(scheme: str, rows: int, ops, batch_sizes, wd: int, bs: int, window: int, form: str) -> float:
@@ -134,9 +194,11 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--stream-batches", type=int, default=64,
                     help="also tune bucket \"stream\" on this many batches of the batched stream (0 = skip)")
+    ap.add_argument("--forms", default="persistent", help="persistent, plain or both (comma-separated)")
     args = ap.parse_args(argv)
+    forms = [f for f in args.forms.split(",") if f]
     table = load_table(args.out)
-    if args.stream_batches:
+    if args.stream_batches and "persistent" in forms:
         from . import workloads
         full = workloads.config4(rows=args.rows)
         nb = min(args.stream_batches, len(full.batch_sizes))
@@ -153,9 +215,18 @@ def main(argv=None) -> int:
             w = generate(args.rows, args.ntx, theta=THETAS[tb], read_ratio=READS[rb], seed=SEED)
             sizes = [min(args.batch, args.ntx - s) for s in range(0, args.ntx, args.batch)]
             for scheme in args.schemes.split(","):
-                best = tune(scheme, args.rows, w, sizes, log=lambda s: print(s, flush=True))
-                table.setdefault(scheme, {})[f"t{tb}-r{rb}"] = best
-                print(json.dumps({"scheme": scheme, "bucket": f"t{tb}-r{rb}", "best": best}), flush=True)
+                if "persistent" in forms:
+                    best = tune(scheme, args.rows, w, sizes, log=lambda s: print(s, flush=True))
+                    best["source"] = (f"autotune coordinate descent on {args.ntx} tx x 16 ops, theta "
+                                      f"{THETAS[tb]}, read ratio {READS[rb]}, batch {args.batch}")
+                    table.setdefault(scheme, {})[f"t{tb}-r{rb}"] = best
+                    print(json.dumps({"scheme": scheme, "bucket": f"t{tb}-r{rb}", "best": best}), flush=True)
+                if "plain" in forms:
+                    best = tune_plain(scheme, args.rows, w, log=lambda s: print(s, flush=True))
+                    best["source"] = (f"autotune coordinate descent (plain form) on {args.ntx} tx x 16 ops, theta "
+                                      f"{THETAS[tb]}, read ratio {READS[rb]}")
+                    table.setdefault(scheme, {})[f"plain-t{tb}-r{rb}"] = best
+                    print(json.dumps({"scheme": scheme, "bucket": f"plain-t{tb}-r{rb}", "best": best}), flush=True)
                 with open(args.out, "w") as f:
                     json.dump(table, f, indent=1, sort_keys=True)
     return 0

@@ -41,7 +41,8 @@ DEVICE_SRCS = [
     "device/exec_row_evt.hip",
     "device/microbench.hip",
 ]
-HOST_SRCS = ["host/engine.cpp", "host/workload.cpp", "host/check.cpp"]
+HOST_SRCS = ["host/engine.cpp", "host/workload.cpp", "host/check.cpp", "host/output.cpp",
+             "host/cpu_path.cpp"]
 TOOLS = {"test_ycsb": "tools/test_ycsb.cpp", "gen_ycsb": "tools/gen_ycsb.cpp"}
 ORACLE_SRCS = ["oracle_core.c", "cpu_cc.c"]
 

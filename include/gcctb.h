@@ -267,6 +267,39 @@ int gcctb_read_txs_file(const char* path, uint64_t ntx, uint32_t cols, uint64_t 
                         uint64_t* tx_off, uint64_t* nops);
 int gcctb_write_txs_file(const char* path, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx);
 
+/* The driver's result block, replacing ExecInfo::Analyse
+ * (include/runtime.cuh:169-214): csv gets the 15-field line the driver
+ * appends to <dataset>.txt, text the stdout block, both with the reference's
+ * arithmetic (float scaling of the tick sums by 1 / (txcnt * kHz), float
+ * milliseconds) and iostream formatting.  Either buffer may be null;
+ * GCCTB_EINVAL if one is too small (including its terminating zero). */
+int gcctb_format_result(const gcctb_stats* st, const char* cc_type, int32_t valid_txn_bitoffset, int32_t warp_cnt,
+                        int32_t batch_size, uint64_t txcnt, char* csv, uint64_t csv_len, char* text,
+                        uint64_t text_len);
+
+/* The CPU path: the eight schemes run by the host's cores, batch after batch
+ * like gcctb_run_plain (benchmark_cpu/ycsb.cu:249-291 with the transaction
+ * body of benchmark/ycsb.cu:51-106), for running the driver's configuration
+ * on a CPU.  Chosen explicitly (test_ycsb: GCCTB_DEVICE=cpu); no GPU entry
+ * point ever falls back to it.  cfg gives cc_scheme, batch_size, rows, cols,
+ * seed and row_ops (warp_density and block_size do not apply); keys address
+ * rows directly, and a key >= rows is an index miss.  table holds rows * cols
+ * cells on the host (gcctb_cpu_seed_table fills it as gcctb_seed_table does
+ * on the device) and is updated in place; reads gets one value per op (0 for
+ * writes) and order three keys per transaction (batch, key0, key1) as
+ * gcctb_fetch returns them.  threads <= 0: every core (at most
+ * OMP_NUM_THREADS when that is set).  Timings are in nanosecond ticks
+ * (st->tick_khz = 1e6). */
+int gcctb_cpu_seed_table(uint64_t* table, uint64_t rows, uint32_t cols, uint64_t seed);
+int gcctb_cpu_run(const gcctb_config* cfg, int32_t threads, uint64_t* table, const uint64_t* ops,
+                  const uint64_t* tx_off, uint64_t ntx, uint64_t* reads, uint64_t* order, gcctb_stats* st);
+/* The serial-replay check of gcctb_check (src/runtime.cu:129-158) on host
+ * buffers, for the CPU path: *bad = reads and cells that differ from
+ * re-executing the transactions one by one in `order` from init_table. */
+int gcctb_check_host(const gcctb_config* cfg, const uint64_t* init_table, const uint64_t* final_table,
+                     const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
+                     const uint64_t* reads, uint64_t* bad);
+
 /* Device information. */
 int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int32_t* cus);
 

@@ -158,3 +158,55 @@ def test_conflict_check_long_chain_no_recursion():
         rows.append((t, 2, 0))
     r = g.conflict_check(_events(rows), n, report_len=4096)
     assert r["cycle"] == [] and r["uncommitted"] == 0
+
+
+def test_output_line_golden():
+    """The driver's result block (ExecInfo::Analyse, include/runtime.cuh:169-214)
+    from fixed statistics, against the text in tests/golden/output_lines.json
+    (one case worked by hand, the rest from an independent float32
+    restatement of the reference's arithmetic and iostream formatting)."""
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "output_lines.json")) as f:
+        cases = json.load(f)
+    assert len(cases) >= 5
+    for c in cases:
+        st = dict(c["stats"])
+        csv, text = g.format_result(st, c["cc_type"], c["warp_density"], c["block_size"], c["batch_size"],
+                                    c["txcnt"])
+        assert csv == c["csv"], (c["cc_type"], csv)
+        assert text == c["text"], (c["cc_type"], text)
+        assert len(csv.strip().split(",")) == 15
+
+
+def test_txs_reader_hand_packed_records():
+    """Records packed here byte by byte in the reference's YCSBTx layout
+    (include/ycsb.cuh:21-33: u64 request_cnt, then 16 x {i32 read, u32 key}),
+    independent of the project's writer: counts, keys and read flags come
+    back in order; request_cnt 0 and 16 are both legal."""
+    import struct
+    recs = [(3, [(1, 7), (0, 4000000000), (1, 0)]), (0, []), (16, [(k % 2, 100 + k) for k in range(16)]),
+            (1, [(0, 42)])]
+    blob = b""
+    for cnt, reqs in recs:
+        body = b"".join(struct.pack("<iI", rd, key) for rd, key in reqs)
+        blob += struct.pack("<Q", cnt) + body + b"\0" * (8 * (16 - len(reqs)))
+    assert len(blob) == 136 * len(recs)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "hand.txs")
+        with open(p, "wb") as f:
+            f.write(blob)
+        r = g.read_txs(p, len(recs), cols=13)
+        with pytest.raises(g.GcctbError):
+            g.read_txs(p, len(recs) + 1, cols=13)  # short file
+        bad = bytearray(blob)
+        bad[136:144] = struct.pack("<Q", 17)
+        with open(p, "wb") as f:
+            f.write(bytes(bad))
+        with pytest.raises(g.GcctbError):
+            g.read_txs(p, len(recs), cols=13)  # request_cnt above 16
+    assert r.tx_off.tolist() == [0, 3, 3, 19, 20]
+    keys = [key for _, reqs in recs for _, key in reqs]
+    writes = [rd == 0 for _, reqs in recs for rd, _ in reqs]
+    assert g.op_key(r.ops).tolist() == keys
+    assert g.op_is_write(r.ops).tolist() == writes
+    assert np.all(g.op_col(r.ops) < 13)
