@@ -86,6 +86,19 @@ class Workload(C.Structure):
     ]
 
 
+class StaticRun(C.Structure):
+    """gcctb_static_run: the plain form over caller-owned device buffers."""
+    _fields_ = [
+        ("cc_scheme", C.c_char_p), ("warp_density", C.c_int32), ("block_size", C.c_int32),
+        ("batch_size", C.c_uint64), ("cells", C.c_void_p), ("rows", C.c_uint64), ("cols", C.c_uint32),
+        ("seed", C.c_uint64), ("backoff_ticks", C.c_uint32), ("ops", C.c_void_p), ("tx_off", C.c_void_p),
+        ("ntx", C.c_uint64), ("nops", C.c_uint64), ("reads", C.c_void_p), ("order", C.c_void_p),
+        ("stream", C.c_void_p),
+    ]
+
+
+STATIC_SCRATCH_BYTES = 24156672  # GCCTB_STATIC_SCRATCH_BYTES
+
 # gcctb_event: a read (0) / write (1) of object oid by transaction tid, or its commit (2)
 EVENT_DTYPE = np.dtype([("id", "<u8"), ("oid", "<u8"), ("tid", "<i4"), ("type", "<i4")])
 
@@ -140,6 +153,7 @@ def load_library() -> C.CDLL:
         "gcctb_microbench": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]),
         "gcctb_sort_pairs": (C.c_int, [P, P, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+        "gcctb_run_plain_static": (C.c_int, [C.POINTER(StaticRun), C.POINTER(Stats)]),
         "gcctb_conflict_check": (C.c_int, [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), P, C.c_uint64,
                                            C.POINTER(C.c_uint64), C.c_char_p, C.c_uint64]),
         "gcctb_fetch_events": (C.c_int, [P, P, C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -470,6 +484,25 @@ class Context:
         w = np.zeros(self.rows, dtype=np.uint64)
         _check(self._lib.gcctb_fetch_cc(self._h, w))
         return w
+
+
+def run_plain_static(cc_scheme: str, *, cells: int, rows: int, cols: int, ops: int, tx_off: int, ntx: int,
+                     nops: int, reads: int, order: int, warp_density: int = 3, block_size: int = 8,
+                     batch_size: int = 1 << 30, seed: int = SEED, backoff_ticks: int = 20, stream: int = 0,
+                     check: bool = True) -> dict:
+    """gcctb_run_plain_static over device pointers (integers, e.g. a torch
+    tensor's data_ptr()): the plain form that allocates nothing.  Returns the
+    stats as a dict with the return code under "rc"."""
+    lib = load_library()
+    r = StaticRun(cc_scheme.encode(), warp_density, block_size, batch_size, cells, rows, cols, seed, backoff_ticks,
+                  ops, tx_off, ntx, nops, reads, order, stream or None)
+    st = Stats()
+    rc = lib.gcctb_run_plain_static(C.byref(r), C.byref(st))
+    if check:
+        _check(rc)
+    d = st.as_dict()
+    d["rc"] = rc
+    return d
 
 
 def conflict_check(events: np.ndarray, ntx: int, report_len: int = 1 << 20) -> dict:

@@ -40,6 +40,7 @@ DEVICE_SRCS = [
     "device/exec_evt.hip",
     "device/exec_row_evt.hip",
     "device/microbench.hip",
+    "device/noalloc.hip",
 ]
 HOST_SRCS = ["host/engine.cpp", "host/workload.cpp", "host/check.cpp", "host/output.cpp",
              "host/cpu_path.cpp"]

@@ -182,6 +182,44 @@ typedef struct gcctb_serve_stats {
 int gcctb_serve(gcctb_ctx* ctx, const gcctb_serve_config* cfg, const uint64_t* batch_sizes,
                 const double* arrival_ms, uint64_t nbatches, double* tx_latency_ms, gcctb_serve_stats* out);
 
+/* The plain form in one call that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * runs the gcctb_run_plain kernels (per batch: counter reset, one launch,
+ * restore of the touched words, host synchronization) for tpl_nw, tpl_wd, to,
+ * mvcc, silo and tictoc over device buffers the caller owns; keys are row ids
+ * (key < rows, the reference's identity id column; other keys are index
+ * misses).  Every other device word (CC words, MVCC chain heads and version
+ * pool, the row of each op, per-transaction words, counters, metrics) lives
+ * in one static device array of GCCTB_STATIC_SCRATCH_BYTES per GPU, held until
+ * the process ends.  Limits: rows <= 1,048,576, ntx <= 131,072,
+ * nops <= 1,048,576 (GCCTB_EINVAL beyond), at most 262,144 MVCC writes per
+ * batch (GCCTB_EKERNEL, version-pool-full, beyond).  Inputs are not checked
+ * beyond these sizes: the ops of a transaction must name distinct keys and be
+ * at most 32.  Every call clears the array's state first; calls that overlap
+ * on one GPU share the array and race. */
+#define GCCTB_STATIC_SCRATCH_BYTES 24156672ull
+typedef struct gcctb_static_run {
+  const char* cc_scheme;    /* tpl_nw tpl_wd to mvcc silo tictoc */
+  int32_t warp_density;     /* 0..5, as in gcctb_config */
+  int32_t block_size;       /* 1..32 */
+  uint64_t batch_size;      /* >= 1 */
+  uint64_t* cells;          /* device: rows x cols u64 cells, updated in place */
+  uint64_t rows;
+  uint32_t cols;            /* 1..255 */
+  uint64_t seed;            /* seeds the written values, as in gcctb_config */
+  uint32_t backoff_ticks;   /* as in gcctb_config */
+  const uint64_t* ops;      /* device: CSR ops, nops entries */
+  const uint64_t* tx_off;   /* device: ntx + 1 offsets */
+  uint64_t ntx;
+  uint64_t nops;
+  uint64_t* reads;          /* device, nops u64: each read op's value (write entries untouched) */
+  uint64_t* order;          /* device, 2 x ntx u64: order[t] = key0, order[ntx + t] = key1; the
+                               serialization order is ascending (t / batch_size, key0, key1) */
+  void* stream;             /* hipStream_t to run on (null: the null stream) */
+} gcctb_static_run;
+int gcctb_run_plain_static(const gcctb_static_run* run, gcctb_stats* out);
+
 /* Uneven batches for the plain and graph forms: sizes must add up to the
  * loaded transaction count (replaces the uniform split of Init0,
  * include/cc.cuh:29-35).  Loading new ops clears it. */

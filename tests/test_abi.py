@@ -104,6 +104,21 @@ def test_check_ops_rejects_repeated_keys():
     assert g.check_ops(g.Ops(two, np.array([0, 1, 2], dtype=np.uint64))) == 1
 
 
+def test_static_entry_point_checks_before_device_use():
+    """gcctb_run_plain_static rejects sizes beyond its static array and the
+    schemes it does not run before it touches a device."""
+    kw = dict(cells=1, cols=10, ops=1, tx_off=1, ntx=1, nops=1, reads=1, order=1)
+    for scheme, rows, extra in [("silo", (1 << 20) + 1, {}), ("gacco", 16, {}), ("gputx", 16, {}),
+                                ("tpl_nw", 16, {"ntx": (1 << 17) + 1}), ("to", 16, {"nops": (1 << 20) + 1}),
+                                ("mvcc", 16, {"warp_density": 6}), ("nope", 16, {})]:
+        args = dict(kw, **extra)
+        st = g.run_plain_static(scheme, rows=rows, check=False, **args)
+        assert st["rc"] == EINVAL, (scheme, rows, extra)
+    if not os.path.exists("/dev/kfd"):
+        st = g.run_plain_static("silo", rows=16, check=False, **kw)
+        assert st["rc"] == -3  # GCCTB_EDEVICE: no GPU, no fallback
+
+
 def _events(rows):
     return np.array([(i + 1, oid, tid, typ) for i, (tid, typ, oid) in enumerate(rows)], dtype=g.EVENT_DTYPE)
 
