@@ -30,7 +30,7 @@ hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t se
 hipError_t index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, uint32_t* buckets,
                         uint32_t nbuckets, hipStream_t s);
 // rows_out[i] = row of ops[i]'s key; also into rows_copy when it is given;
-// ids_out[i] = (id_base + i) | OPID_WRITE if ops[i] writes, when it is given
+// ids_out[i] = what op_ids(ops, n, id_base) writes, when it is given
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
                  hipStream_t s, uint32_t* rows_copy = nullptr, uint32_t* ids_out = nullptr,
                  uint32_t id_base = 0);
@@ -38,6 +38,9 @@ hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t*
 // --- per-op helpers ---
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s);
 hipError_t iota_u32(uint32_t* v, uint64_t n, uint32_t base, hipStream_t s);
+// out[i] = (base + i) | OPID_WRITE if ops[base + i] writes: the values the
+// GaccO / GPUTx sorts carry, so later passes read the flag in order.
+hipError_t op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out, hipStream_t s);
 hipError_t fill_u32(uint32_t* v, uint64_t n, uint32_t x, hipStream_t s);
 hipError_t fill_u64(uint64_t* v, uint64_t n, uint64_t x, hipStream_t s);
 
@@ -59,7 +62,7 @@ hipError_t inclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint
 hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
 
 // --- ordered-execution preprocessing ---
-// From accesses sorted by (row, order), sop holding op id values: group-start
+// From accesses sorted by (row, order), sop holding op_ids values: group-start
 // flags (p or 0) and, when rowstart is given, row-start flags (p or 0).
 hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags,
                               hipStream_t s, uint32_t* rowstart = nullptr);
@@ -90,30 +93,15 @@ hipError_t gtx_conflicts32(const uint32_t* skey, const uint32_t* flags, uint64_t
 // starts else 0.
 hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint64_t nrows, uint32_t* boundary,
                      uint32_t* rowstart, hipStream_t s);
-// local rank = S[p] - S[SS[p]]; rank[tx] = max; max_rank = max; ptx[p] = the
-// transaction of the access at p; big[p] = 1 if its row has more than small
-// accesses (an index miss never counts).
-hipError_t gtx_rank(const uint32_t* srow, const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
-                    uint64_t nrows, uint32_t small, const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank,
-                    uint32_t* ptx, uint32_t* big, hipStream_t s);
-// need[op] for the accesses of rows with at most small accesses, in (rank,
-// transaction id) order per row, counted from the row's first access;
-// *conflicts += their same-rank conflicts.
-hipError_t gtx_need_small(const uint32_t* srow, const uint32_t* sop, const uint32_t* SS, const uint32_t* ptx,
-                          const uint32_t* rank, uint64_t n, uint64_t nrows, uint32_t small, uint32_t* need,
-                          uint64_t* conflicts, hipStream_t s);
-// the accesses with big[p] set (cbig = inclusive sum of big) moved to the
-// front in order: row, op id, rank
-hipError_t gtx_compact(const uint32_t* big, const uint32_t* cbig, const uint32_t* srow, const uint32_t* sop,
-                       const uint32_t* ptx, const uint32_t* rank, uint64_t n, uint32_t* sub_row, uint32_t* sub_op,
-                       uint32_t* sub_rank, hipStream_t s);
-// heads[i] = 1 where srow changes (and at 0)
-hipError_t row_heads(const uint32_t* srow, uint64_t n, uint32_t* heads, hipStream_t s);
-// key[i] = ((segc[i] - 1) << rank_bits) | sub_rank[i]; key may alias sub_rank
-hipError_t gtx_sub_keys32(const uint32_t* segc, const uint32_t* sub_rank, uint64_t n, int rank_bits, uint32_t* key,
-                          hipStream_t s);
-hipError_t gtx_sub_keys64(const uint32_t* segc, const uint32_t* sub_rank, uint64_t n, int rank_bits, uint64_t* key,
-                          hipStream_t s);
+// local rank = S[p] - S[SS[p]]; rank[tx] = max; max_rank = max.
+hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                    const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s);
+// composite key (row << rank_bits) | rank[op_tx[op]] for ops [0, n)
+hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                         uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s);
+// the same key cut to 32 bits, for row_bits + rank_bits <= 32
+hipError_t gtx_exec_keys32(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                           uint64_t n, int rank_bits, uint32_t* keys, hipStream_t s);
 // out[i] = keys[i] >> shift
 hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s);
 hipError_t u32_key_high(const uint32_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s);

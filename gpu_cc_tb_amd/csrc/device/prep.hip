@@ -186,6 +186,13 @@ __global__ void k_op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx) {
     for (uint64_t i = b; i < e; ++i) op_tx[i] = (uint32_t)t;
   }
 }
+// Op ids for the GaccO / GPUTx sorts: base + i with the op's write flag in
+// bit 31, so the passes over the sorted order read it without a random load.
+__global__ void k_op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (base + (uint32_t)i) | (op_is_write(ops[base + i]) ? OPID_WRITE : 0u);
+}
 __global__ void k_iota(uint32_t* v, uint64_t n, uint32_t base) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -581,121 +588,25 @@ __global__ void k_gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t 
                    ((sop[p] | sop[p - 1]) & OPID_WRITE) != 0u) ? 1u : 0u;
   }
 }
-// rank[tx] = max local rank of its accesses; also, per sorted position p,
-// ptx[p] = the transaction of the access and big[p] = 1 when the access's
-// row has more than `small` accesses (such rows are put in execution order
-// by a sort; the others by k_gtx_need_small, in place).
-__global__ void k_gtx_rank(const uint32_t* srow, const uint32_t* sop, const uint32_t* S, const uint32_t* SS,
-                           uint64_t n, uint64_t nrows, uint32_t small, const uint32_t* op_tx, uint32_t* rank,
-                           uint32_t* max_rank, uint32_t* ptx, uint32_t* big) {
+__global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                           const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank) {
   uint32_t mx = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t s = SS[p];
-    const uint32_t local = S[p] - S[s];
-    const uint32_t tx = op_tx[sop[p] & OPID_MASK];
-    if (local) atomicMax(&rank[tx], local);  // ranks start at 0
+    const uint32_t local = S[p] - S[SS[p]];
+    if (local) atomicMax(&rank[op_tx[sop[p] & OPID_MASK]], local);  // ranks start at 0
     mx = local > mx ? local : mx;
-    ptx[p] = tx;
-    const uint32_t r = srow[p];
-    big[p] = (r < nrows && (uint64_t)s + small < n && srow[(uint64_t)s + small] == r) ? 1u : 0u;
   }
   // one atomic per block (launched with reduce_blocks)
   mx = (uint32_t)block_reduce_u64<true>(mx);
   if (threadIdx.x == 0 && mx) atomicMax(max_rank, mx);
 }
-
-// Turns of the accesses of rows with at most `small` accesses, in GPUTx's
-// execution order of a row, (rank, transaction id).  A row's accesses sit at
-// [SS[p], end) in (row, transaction id) order; the thread of access p counts
-// those that execute before it (a lower rank, or the same rank and an earlier
-// position).  Its turn: its own position if it writes; if it reads, one past
-// the position of the last write before it (the reads between two writes
-// share a turn), or 0.  The turns count from the row's first access, so the
-// counters start at zero.  Same-rank conflicts: p and the access just before
-// it in the order have one rank and either writes (counted at p).
-__global__ void k_gtx_need_small(const uint32_t* srow, const uint32_t* sop, const uint32_t* SS, const uint32_t* ptx,
-                                 const uint32_t* rank, uint64_t n, uint64_t nrows, uint32_t small, uint32_t* need,
-                                 unsigned long long* conflicts) {
-  uint64_t found = 0;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = srow[p];
-    if (r >= nrows) continue;  // an index miss never executes
-    const uint64_t s = SS[p];
-    if (s + small < n && srow[s + small] == r) continue;  // a long row: ordered by the sort
-    const uint64_t e = s + small < n ? s + small : n;     // the row ends at or before e
-    const uint32_t op = sop[p];
-    const bool w = (op & OPID_WRITE) != 0u;
-    const uint32_t rk = rank[ptx[p]];
-    uint32_t pos = 0, pr = 0, wr = 0;
-    uint64_t pq = 0, wq = 0;
-    bool have_pred = false, pred_w = false, have_w = false;
-    for (uint64_t q = s; q < e && srow[q] == r; ++q) {
-      const uint32_t rq = rank[ptx[q]];
-      if (!(rq < rk || (rq == rk && q < p))) continue;
-      ++pos;
-      const bool qw = (sop[q] & OPID_WRITE) != 0u;
-      // keys (rank, position) only grow along the order
-      if (!have_pred || rq > pr || (rq == pr && q > pq)) {
-        have_pred = true;
-        pr = rq;
-        pq = q;
-        pred_w = qw;
-      }
-      if (qw && (!have_w || rq > wr || (rq == wr && q > wq))) {
-        have_w = true;
-        wr = rq;
-        wq = q;
-      }
-    }
-    uint32_t turn = pos;
-    if (!w) {
-      turn = 0;
-      if (have_w) {
-        uint32_t wpos = 0;
-        for (uint64_t q = s; q < e && srow[q] == r; ++q) {
-          const uint32_t rq = rank[ptx[q]];
-          wpos += (rq < wr || (rq == wr && q < wq)) ? 1u : 0u;
-        }
-        turn = wpos + 1;
-      }
-    }
-    need[op & OPID_MASK] = turn;
-    found += (have_pred && pr == rk && (pred_w || w)) ? 1u : 0u;
-  }
-  // one atomic per block (launched with reduce_blocks)
-  found = block_reduce_u64<false>(found);
-  if (threadIdx.x == 0 && found) atomicAdd(conflicts, (unsigned long long)found);
-}
-
-// The accesses of long rows (big[p] = 1; cbig = inclusive sum of big) moved
-// to [0, n_big) in the same order: row, op id and rank of each.
-__global__ void k_gtx_compact(const uint32_t* big, const uint32_t* cbig, const uint32_t* srow, const uint32_t* sop,
-                              const uint32_t* ptx, const uint32_t* rank, uint64_t n, uint32_t* sub_row,
-                              uint32_t* sub_op, uint32_t* sub_rank) {
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    if (!big[p]) continue;
-    const uint32_t i = cbig[p] - 1u;
-    sub_row[i] = srow[p];
-    sub_op[i] = sop[p];
-    sub_rank[i] = rank[ptx[p]];
-  }
-}
-// heads[i] = 1 where a new row starts
-__global__ void k_row_heads(const uint32_t* srow, uint64_t n, uint32_t* heads) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    heads[i] = (i == 0 || srow[i] != srow[i - 1]) ? 1u : 0u;
-}
-// key[i] = (row number among the long rows << rank_bits) | rank; segc is the
-// inclusive sum of the row heads.  key may alias sub_rank.
 template <class KT>
-__global__ void k_gtx_sub_keys(const uint32_t* segc, const uint32_t* sub_rank, uint64_t n, int rank_bits, KT* key) {
+__global__ void k_gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                                uint64_t n, int rank_bits, KT* keys) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
-    key[i] = (KT)(((uint64_t)(segc[i] - 1u) << rank_bits) | sub_rank[i]);
+    keys[i] = (KT)(((uint64_t)rows[i] << rank_bits) | rank[op_tx[i]]);
 }
 template <class KT>
 __global__ void k_key_high(const KT* keys, uint64_t n, int shift, uint32_t* out) {
@@ -795,6 +706,9 @@ hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStrea
 hipError_t iota_u32(uint32_t* v, uint64_t n, uint32_t base, hipStream_t s) {
   GC_LAUNCH(k_iota, n, v, n, base);
 }
+hipError_t op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out, hipStream_t s) {
+  GC_LAUNCH(k_op_ids, n, ops, n, base, out);
+}
 hipError_t fill_u32(uint32_t* v, uint64_t n, uint32_t x, hipStream_t s) { GC_LAUNCH(k_fill32, n, v, n, x); }
 hipError_t fill_u64(uint64_t* v, uint64_t n, uint64_t x, hipStream_t s) { GC_LAUNCH(k_fill64, n, v, n, x); }
 
@@ -844,32 +758,17 @@ hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint
                      uint32_t* rowstart, hipStream_t s) {
   GC_LAUNCH(k_gtx_flags, n, srow, sop, n, nrows, boundary, rowstart);
 }
-hipError_t gtx_rank(const uint32_t* srow, const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
-                    uint64_t nrows, uint32_t small, const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank,
-                    uint32_t* ptx, uint32_t* big, hipStream_t s) {
-  GC_LAUNCH_REDUCE(k_gtx_rank, n, srow, sop, S, SS, n, nrows, small, op_tx, rank, max_rank, ptx, big);
+hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                    const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s) {
+  GC_LAUNCH_REDUCE(k_gtx_rank, n, sop, S, SS, n, op_tx, rank, max_rank);
 }
-hipError_t gtx_need_small(const uint32_t* srow, const uint32_t* sop, const uint32_t* SS, const uint32_t* ptx,
-                          const uint32_t* rank, uint64_t n, uint64_t nrows, uint32_t small, uint32_t* need,
-                          uint64_t* conflicts, hipStream_t s) {
-  GC_LAUNCH_REDUCE(k_gtx_need_small, n, srow, sop, SS, ptx, rank, n, nrows, small, need,
-                   (unsigned long long*)conflicts);
+hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                         uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s) {
+  GC_LAUNCH(k_gtx_exec_keys<uint64_t>, n, rows, op_tx, rank, n, rank_bits, keys);
 }
-hipError_t gtx_compact(const uint32_t* big, const uint32_t* cbig, const uint32_t* srow, const uint32_t* sop,
-                       const uint32_t* ptx, const uint32_t* rank, uint64_t n, uint32_t* sub_row, uint32_t* sub_op,
-                       uint32_t* sub_rank, hipStream_t s) {
-  GC_LAUNCH(k_gtx_compact, n, big, cbig, srow, sop, ptx, rank, n, sub_row, sub_op, sub_rank);
-}
-hipError_t row_heads(const uint32_t* srow, uint64_t n, uint32_t* heads, hipStream_t s) {
-  GC_LAUNCH(k_row_heads, n, srow, n, heads);
-}
-hipError_t gtx_sub_keys32(const uint32_t* segc, const uint32_t* sub_rank, uint64_t n, int rank_bits, uint32_t* key,
-                          hipStream_t s) {
-  GC_LAUNCH(k_gtx_sub_keys<uint32_t>, n, segc, sub_rank, n, rank_bits, key);
-}
-hipError_t gtx_sub_keys64(const uint32_t* segc, const uint32_t* sub_rank, uint64_t n, int rank_bits, uint64_t* key,
-                          hipStream_t s) {
-  GC_LAUNCH(k_gtx_sub_keys<uint64_t>, n, segc, sub_rank, n, rank_bits, key);
+hipError_t gtx_exec_keys32(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
+                           uint64_t n, int rank_bits, uint32_t* keys, hipStream_t s) {
+  GC_LAUNCH(k_gtx_exec_keys<uint32_t>, n, rows, op_tx, rank, n, rank_bits, keys);
 }
 hipError_t u64_key_high(const uint64_t* keys, uint64_t n, int shift, uint32_t* out, hipStream_t s) {
   GC_LAUNCH(k_key_high<uint64_t>, n, keys, n, shift, out);

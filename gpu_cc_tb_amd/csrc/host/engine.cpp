@@ -121,7 +121,7 @@ struct gcctb_ctx {
   gcctb::DevBuf<uint32_t> tx_aborts;
   gcctb::DevBuf<gcctb::Metrics> metrics;
   // preprocessing scratch
-  gcctb::DevBuf<uint32_t> sk0, sv0, sk1, sv1, flags, gstart, sort_tmp, S, SS, ptx, big;
+  gcctb::DevBuf<uint32_t> sk0, sv0, sk1, sv1, flags, gstart, sort_tmp, S, SS;
   gcctb::DevBuf<uint64_t> sk64a, sk64b;
   gcctb::DevBuf<uint32_t> rank, tx_map, max_rank;
   gcctb::DevBuf<uint64_t> conflicts;
@@ -450,108 +450,77 @@ class GPUTxHost : public SchemeHost {
     c.rank_counts_ready = true;
   }
   // rank[] of every transaction, tx_map in (rank, tx) order, and need[] /
-  // counters[] for execution in (row, rank, tx) order.  One sort by row puts
-  // every row's accesses together in transaction order; the ranks come from
-  // that order (gputx_pre2, src/gputx.cu:5-27).  The execution order differs
-  // from it only inside a row, by rank: a row of at most SMALL_ROW accesses
-  // is put in order by the threads of its accesses, each counting the
-  // accesses that precede it; the accesses of longer rows are moved aside
-  // and sorted by (row, rank), a sort of those accesses alone.
-  static constexpr uint32_t SMALL_ROW = 32;
+  // counters[] for execution in (row, rank, tx) order
   void rank_all() {
     ctx.rank_counts_ready = false;
-    const uint64_t n = ctx.nops, ntx = ctx.ntx, nrows = ctx.cfg.rows;
+    const uint64_t n = ctx.nops, ntx = ctx.ntx;
     hipStream_t s = ctx.stream;
     // per-row access lists in (row, tx) order
     HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p, ctx.sv0.p, 0));
     uint32_t *srow, *sop;
     // rows are below cfg.rows; a miss (IDX_MISS) has all row bits set and sorts last
-    const int row_bits = bits_for(nrows);
+    const int row_bits = bits_for(ctx.cfg.rows);
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, row_bits, ctx.sort_tmp.p,
                                    &srow, &sop, s));
-    // the sort's other key and value buffers
-    uint32_t* bk = srow == ctx.sk0.p ? ctx.sk1.p : ctx.sk0.p;
-    uint32_t* bv = sop == ctx.sv0.p ? ctx.sv1.p : ctx.sv0.p;
     // one-pass local ranks and rank[tx] = max (src/gputx.cu:5-27)
-    HIP_TRY(launch::gtx_flags(srow, sop, n, nrows, ctx.flags.p, ctx.gstart.p, s));
+    HIP_TRY(launch::gtx_flags(srow, sop, n, ctx.cfg.rows, ctx.flags.p, ctx.gstart.p, s));
     HIP_TRY(launch::inclusive_sum_u32(ctx.flags.p, ctx.S.p, n, ctx.sort_tmp.p, s));
     HIP_TRY(launch::inclusive_max_u32(ctx.gstart.p, ctx.SS.p, n, ctx.sort_tmp.p, s));
     HIP_TRY(launch::fill_u32(ctx.rank.p, ntx, 0, s));
     HIP_TRY(launch::fill_u32(ctx.max_rank.p, 1, 0, s));
-    HIP_TRY(launch::gtx_rank(srow, sop, ctx.S.p, ctx.SS.p, n, nrows, SMALL_ROW, ctx.op_tx.p, ctx.rank.p,
-                             ctx.max_rank.p, ctx.ptx.p, ctx.big.p, s));
-    uint32_t* cbig = ctx.gstart.p;  // free once SS is made
-    HIP_TRY(launch::inclusive_sum_u32(ctx.big.p, cbig, n, ctx.sort_tmp.p, s));
-    uint32_t host[2] = {0, 0};  // max rank, accesses of long rows
-    HIP_TRY(hipMemcpyAsync(&host[0], ctx.max_rank.p, 4, hipMemcpyDeviceToHost, s));
-    if (n) HIP_TRY(hipMemcpyAsync(&host[1], cbig + n - 1, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(launch::gtx_rank(sop, ctx.S.p, ctx.SS.p, n, ctx.op_tx.p, ctx.rank.p, ctx.max_rank.p, s));
+    uint32_t max_rank = 0;
+    HIP_TRY(hipMemcpyAsync(&max_rank, ctx.max_rank.p, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    const uint64_t n_big = host[1];
     // no ops at all: every transaction has rank 0
-    ctx.nranks = n ? (uint64_t)host[0] + 1 : (ntx ? 1 : 0);
-    // transactions in (rank, tx) order: stable sort of ranks (include/gputx.cuh:281);
-    // srow / sop stay where they are
+    ctx.nranks = n ? (uint64_t)max_rank + 1 : (ntx ? 1 : 0);
+    // transactions in (rank, tx) order: stable sort of ranks (include/gputx.cuh:281)
     const int rank_bits = std::max(1, bits_for(ctx.nranks ? ctx.nranks - 1 : 0));
-    HIP_TRY(hipMemcpyAsync(bk, ctx.rank.p, ntx * 4, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(launch::iota_u32(bv, ntx, 0, s));
+    HIP_TRY(hipMemcpyAsync(ctx.sk0.p, ctx.rank.p, ntx * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch::iota_u32(ctx.sv0.p, ntx, 0, s));
     uint32_t *srank, *stx;
-    HIP_TRY(launch::radix_sort_u32(bk, bv, ctx.flags.p, ctx.S.p, ntx, rank_bits, ctx.sort_tmp.p, &srank, &stx, s));
+    HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, ntx, rank_bits, ctx.sort_tmp.p,
+                                   &srank, &stx, s));
     HIP_TRY(hipMemcpyAsync(ctx.tx_map.p, stx, ntx * 4, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
-    if (n) HIP_TRY(launch::gtx_need_small(srow, sop, ctx.SS.p, ctx.ptx.p, ctx.rank.p, n, nrows, SMALL_ROW,
-                                          ctx.need.p, ctx.conflicts.p, s));
-    if (n_big) order_long_rows(srow, sop, cbig, bk, bv, n_big, rank_bits);
-    HIP_TRY(launch::fill_u32(ctx.counters.p, ctx.counters.n, 0, s));
+    // execution order of every row: (row, rank, tx); 32-bit composite keys
+    // when row and rank fit, else 64-bit
+    if (n && row_bits + rank_bits <= 32) {
+      HIP_TRY(launch::gtx_exec_keys32(ctx.rows_out.p, ctx.op_tx.p, ctx.rank.p, n, rank_bits, ctx.sk0.p, s));
+      HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
+      uint32_t *skey, *sop2;
+      HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, row_bits + rank_bits,
+                                     ctx.sort_tmp.p, &skey, &sop2, s));
+      uint32_t* srow2 = ctx.S.p;  // free once the ranks are made
+      HIP_TRY(launch::u32_key_high(skey, n, rank_bits, srow2, s));
+      ordered_need(srow2, sop2);
+      HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
+      HIP_TRY(launch::gtx_conflicts32(skey, ctx.flags.p, n, ctx.conflicts.p, s));
+    } else if (n) {
+      HIP_TRY(launch::gtx_exec_keys(ctx.rows_out.p, ctx.op_tx.p, ctx.rank.p, n, rank_bits, ctx.sk64a.p, s));
+      HIP_TRY(launch::op_ids(ctx.ops.p, n, 0, ctx.sv0.p, s));
+      uint64_t* skey;
+      uint32_t* sop2;
+      HIP_TRY(launch::radix_sort_u64(ctx.sk64a.p, ctx.sv0.p, ctx.sk64b.p, ctx.sv1.p, n, row_bits + rank_bits,
+                                     ctx.sort_tmp.p, &skey, &sop2, s));
+      HIP_TRY(launch::u64_key_high(skey, n, rank_bits, ctx.sk0.p, s));
+      ordered_need(ctx.sk0.p, sop2);
+      HIP_TRY(launch::fill_u64(ctx.conflicts.p, 1, 0, s));
+      HIP_TRY(launch::gtx_conflicts(skey, ctx.flags.p, n, ctx.conflicts.p, s));
+    }
   }
-  // The accesses of rows longer than SMALL_ROW, moved to the front in (row,
-  // tx) order and sorted by (row number among them, rank): group starts,
-  // turns relative to the row's first access, same-rank conflicts.
-  void order_long_rows(const uint32_t* srow, const uint32_t* sop, const uint32_t* cbig, uint32_t* bk, uint32_t* bv,
-                       uint64_t m, int rank_bits) {
+  // need[] from accesses in execution order (group flags in ctx.flags, which
+  // the conflict count reads afterwards).  A row occupies the same positions
+  // in the (row, tx) and (row, rank, tx) orders, so ctx.SS (the row start of
+  // every position, made for the ranks) makes each turn relative to its row,
+  // and the counters start at zero: one dense fill instead of a random store
+  // per row.
+  void ordered_need(const uint32_t* srow, const uint32_t* sop) {
     const uint64_t n = ctx.nops;
     hipStream_t s = ctx.stream;
-    uint32_t *sub_row = bk, *sub_op = bv, *sub_rank = ctx.S.p;
-    HIP_TRY(launch::gtx_compact(ctx.big.p, cbig, srow, sop, ctx.ptx.p, ctx.rank.p, n, sub_row, sub_op, sub_rank, s));
-    // row number among the long rows (1-based), in ptx (free now)
-    HIP_TRY(launch::row_heads(sub_row, m, ctx.flags.p, s));
-    uint32_t* segc = ctx.ptx.p;
-    HIP_TRY(launch::inclusive_sum_u32(ctx.flags.p, segc, m, ctx.sort_tmp.p, s));
-    // at most m / (SMALL_ROW + 1) long rows
-    const int seg_bits = std::max(1, bits_for(m / (SMALL_ROW + 1)));
-    uint32_t* segid = segc;  // the sort's row numbers, 0-based, replace segc
-    const uint32_t *ssop, *gkey32 = nullptr;
-    const uint64_t* gkey64 = nullptr;
-    uint32_t *fa, *fb;  // two free buffers after the sort
-    if (seg_bits + rank_bits <= 32) {
-      uint32_t* key = sub_rank;  // built in place
-      HIP_TRY(launch::gtx_sub_keys32(segc, sub_rank, m, rank_bits, key, s));
-      uint32_t *skey, *sv;
-      HIP_TRY(launch::radix_sort_u32(key, sub_op, ctx.flags.p, sub_row, m, seg_bits + rank_bits, ctx.sort_tmp.p,
-                                     &skey, &sv, s));
-      HIP_TRY(launch::u32_key_high(skey, m, rank_bits, segid, s));
-      ssop = sv;
-      gkey32 = skey;
-      fa = skey == key ? ctx.flags.p : key;
-      fb = sv == sub_op ? sub_row : sub_op;
-    } else {
-      HIP_TRY(launch::gtx_sub_keys64(segc, sub_rank, m, rank_bits, ctx.sk64a.p, s));
-      uint64_t* skey;
-      uint32_t* sv;
-      HIP_TRY(launch::radix_sort_u64(ctx.sk64a.p, sub_op, ctx.sk64b.p, sub_row, m, seg_bits + rank_bits,
-                                     ctx.sort_tmp.p, &skey, &sv, s));
-      HIP_TRY(launch::u64_key_high(skey, m, rank_bits, segid, s));
-      ssop = sv;
-      gkey64 = skey;
-      fa = ctx.flags.p;
-      fb = sv == sub_op ? sub_row : sub_op;
-    }
-    uint32_t *gflags = fa, *rflags = fb;
-    HIP_TRY(launch::access_group_flags(segid, ssop, m, gflags, s, rflags));
-    HIP_TRY(launch::inclusive_max_u32(gflags, ctx.gstart.p, m, ctx.sort_tmp.p, s));
-    HIP_TRY(launch::inclusive_max_u32(rflags, ctx.SS.p, m, ctx.sort_tmp.p, s));
-    HIP_TRY(launch::access_need_rel(ssop, ctx.gstart.p, ctx.SS.p, m, ctx.need.p, s));
-    if (gkey32) HIP_TRY(launch::gtx_conflicts32(gkey32, gflags, m, ctx.conflicts.p, s));
-    else HIP_TRY(launch::gtx_conflicts(gkey64, gflags, m, ctx.conflicts.p, s));
+    HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s));
+    HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
+    HIP_TRY(launch::access_need_rel(sop, ctx.gstart.p, ctx.SS.p, n, ctx.need.p, s));
+    HIP_TRY(launch::fill_u32(ctx.counters.p, ctx.counters.n, 0, s));
   }
   void Init(int, uint64_t) override {}
   // the persistent form runs every rank level in one launch: it needs no
@@ -936,8 +905,6 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
       if (c->scheme == SCHEME_GPUTX) {
         c->S.alloc(n);
         c->SS.alloc(n);
-        c->ptx.alloc(n);
-        c->big.alloc(n);
         c->sk64a.alloc(n);
         c->sk64b.alloc(n);
         c->rank.alloc(ntx);
