@@ -88,6 +88,17 @@ struct TxCtx {
   uint64_t st[K];
 };
 
+// A row's CC word, MVCC chain head and turn counter (at its slot).
+__device__ __forceinline__ uint64_t* cc_word(const ExecArgs& a, uint32_t row) {
+  return a.cc + row_slot(row, a.slot_shift);
+}
+__device__ __forceinline__ uint32_t* chain_head(const ExecArgs& a, uint32_t row) {
+  return a.vprev + row_slot(row, a.slot_shift);
+}
+__device__ __forceinline__ uint32_t* turn_counter(const ExecArgs& a, uint32_t row) {
+  return a.counters + row_slot(row, a.slot_shift);
+}
+
 __device__ __forceinline__ uint64_t* cell_ptr(const ExecArgs& a, uint32_t row, uint32_t meta) {
   return a.cells + (uint64_t)row * a.cols + (meta >> M_COL_SHIFT);
 }
@@ -328,13 +339,13 @@ struct TwoPL {
           cur[k] = 0;
           got[k] = false;
           if ((m & M_VALID) && !(m & M_HELD))
-            got[k] = cas(a.cc + c.row[k], cur[k], (m & M_WRITE) ? tpl::make_excl(c.tx) : tpl::CNT_ONE);
+            got[k] = cas(cc_word(a, c.row[k]), cur[k], (m & M_WRITE) ? tpl::make_excl(c.tx) : tpl::CNT_ONE);
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const uint32_t m = c.meta[k];
           if ((m & M_VALID) && !(m & M_HELD)) {
-            uint64_t* w = a.cc + c.row[k];
+            uint64_t* w = cc_word(a, c.row[k]);
             StepResult r = R_OK;
             if (!got[k]) r = (m & M_WRITE) ? lock_excl(w, c.tx, cur[k]) : lock_shared(w, c.tx, cur[k]);
             if (r == R_OK)
@@ -348,7 +359,7 @@ struct TwoPL {
         if (group_any<G>(conflict)) {
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            if (c.meta[k] & M_HELD) unlock(a.cc + c.row[k], c.meta[k] & M_WRITE);
+            if (c.meta[k] & M_HELD) unlock(cc_word(a, c.row[k]), c.meta[k] & M_WRITE);
           abort_attempt(c, a, now);
         } else if (group_all<G>(!pending)) {
           c.ph = PH_COMMIT;
@@ -394,7 +405,7 @@ struct TwoPL {
       order_release();
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if (c.meta[k] & M_VALID) unlock(a.cc + c.row[k], c.meta[k] & M_WRITE);
+        if (c.meta[k] & M_VALID) unlock(cc_word(a, c.row[k]), c.meta[k] & M_WRITE);
       finish_tx(c, a, now, ticket, 0);
     }
   }
@@ -447,7 +458,7 @@ __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs&
   for (int k = 0; k < K; ++k) {
     const uint32_t m = c.meta[k];
     rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
-    cur[k] = rd[k] ? ld(a.cc + c.row[k]) : 0;
+    cur[k] = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -478,14 +489,14 @@ __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs&
     ok[k] = false;
     if (!cand[k]) continue;
     if (tsw::rts(cur[k]) >= ts)
-      cur[k] = ld(a.cc + c.row[k]);  // nothing to raise: re-check the version
+      cur[k] = ld(cc_word(a, c.row[k]));  // nothing to raise: re-check the version
     else
-      ok[k] = cas(a.cc + c.row[k], cur[k], tsw::make(0, ts, tsw::wts(cur[k])));
+      ok[k] = cas(cc_word(a, c.row[k]), cur[k], tsw::make(0, ts, tsw::wts(cur[k])));
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (!cand[k]) continue;
-    if (!ok[k]) ok[k] = hold_version(a.cc + c.row[k], ts, ver[k], cur[k], true);
+    if (!ok[k]) ok[k] = hold_version(cc_word(a, c.row[k]), ts, ver[k], cur[k], true);
     res[k] = ok[k] ? R_OK : R_WAIT;
   }
 }
@@ -564,7 +575,7 @@ struct TimestampOrdering {
       for (int k = 0; k < K; ++k) {
         const uint32_t m = c.meta[k];
         if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_DONE)) continue;
-        const StepResult r = write_mark(a.cc + c.row[k], c.ts, c.st[k], seen);
+        const StepResult r = write_mark(cc_word(a, c.row[k]), c.ts, c.st[k], seen);
         if (r == R_OK) {
           c.meta[k] = m | M_DONE | M_HELD;
           ev_op<G>(c, a, k);
@@ -579,7 +590,7 @@ struct TimestampOrdering {
       // rollback (to.cuh:255-270): restore the words this attempt marked
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+        if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), c.st[k]);
       raise_ts_floor<G>(a, group_max64<G>(seen));
       abort_attempt(c, a, now);
       return;
@@ -593,7 +604,7 @@ struct TimestampOrdering {
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], tsw::make(0, c.ts, c.ts));
+      if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), tsw::make(0, c.ts, c.ts));
     finish_tx(c, a, now, c.ts, 0);
   }
 };
@@ -618,7 +629,7 @@ struct Mvcc {
   static constexpr bool kBulkRefill = true;
   static __device__ StepResult read(const ExecArgs& a, uint32_t row, uint32_t col, uint64_t ts,
                                     uint64_t& val) {
-    uint64_t* w = a.cc + row;
+    uint64_t* w = cc_word(a, row);
     const uint32_t meta = col << M_COL_SHIFT;
     const uint64_t stride = mvcc_stride(a);
     uint64_t cur = ld(w);
@@ -639,7 +650,7 @@ struct Mvcc {
       // chain head never change, so the walk itself needs no validation and a
       // reader far behind a hot row's head still finishes.
       order_acquire();
-      uint32_t p = ld32(a.vprev + row);
+      uint32_t p = ld32(chain_head(a, row));
       uint64_t v = op_load(a, row, meta);
       order_acquire();
       const uint64_t after = ld(w);
@@ -735,7 +746,7 @@ struct Mvcc {
       for (int k = 0; k < K; ++k) {
         const uint32_t m = c.meta[k];
         if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_DONE)) continue;
-        uint64_t* w = a.cc + c.row[k];
+        uint64_t* w = cc_word(a, c.row[k]);
         uint64_t cur = ld(w);
         StepResult r = R_WAIT;
 #pragma unroll 1
@@ -763,7 +774,7 @@ struct Mvcc {
       if (group_any<G>(abort)) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+          if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), c.st[k]);
         raise_ts_floor<G>(a, group_max64<G>(seen));
         abort_attempt(c, a, now);
       } else if (group_all<G>(!pending)) {
@@ -793,7 +804,7 @@ struct Mvcc {
     for (int k = 0; k < K; ++k) {
       const bool h = (c.meta[k] & M_HELD) != 0;
       oldv[k] = h && !kRowOps ? ld(cell_ptr(a, c.row[k], c.meta[k])) : 0;
-      prev[k] = h ? ld32(a.vprev + c.row[k]) : 0;
+      prev[k] = h ? ld32(chain_head(a, c.row[k])) : 0;
     }
     uint64_t nd_of[K];
 #pragma unroll
@@ -825,12 +836,12 @@ struct Mvcc {
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (nd_of[k] != ~0ull) st32(a.vprev + c.row[k], (uint32_t)(nd_of[k] + 1));
+      if (nd_of[k] != ~0ull) st32(chain_head(a, c.row[k]), (uint32_t)(nd_of[k] + 1));
     ev_commit<G>(c, a);
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], tsw::make(0, c.ts, c.ts));
+      if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), tsw::make(0, c.ts, c.ts));
     finish_tx(c, a, now, c.ts, 0);
   }
 };
@@ -852,7 +863,7 @@ __device__ __forceinline__ void occ_snapshot(const TxCtx<G>& c, const ExecArgs& 
   for (int k = 0; k < K; ++k) {
     const uint32_t m = c.meta[k];
     rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
-    w1[k] = rd[k] ? ld(a.cc + c.row[k]) : 0;
+    w1[k] = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
   }
   order_acquire();
 #pragma unroll
@@ -860,7 +871,7 @@ __device__ __forceinline__ void occ_snapshot(const TxCtx<G>& c, const ExecArgs& 
   order_acquire();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const uint64_t w2 = rd[k] ? ld(a.cc + c.row[k]) : 0;
+    const uint64_t w2 = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
     ok[k] = rd[k] && w1[k] == w2 && !(w1[k] & 1u);
   }
 }
@@ -876,7 +887,7 @@ __device__ __forceinline__ void occ_lock(const TxCtx<G>& c, const ExecArgs& a, u
   for (int k = 0; k < K; ++k) {
     const uint32_t m = c.meta[k];
     tried[k] = (m & M_VALID) && (m & M_WRITE) && !(m & M_HELD);
-    old[k] = tried[k] ? fetch_or(a.cc + c.row[k], 1ull) : 0;
+    old[k] = tried[k] ? fetch_or(cc_word(a, c.row[k]), 1ull) : 0;
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) got[k] = tried[k] && !(old[k] & 1u);
@@ -909,7 +920,7 @@ struct Silo {
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
       rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
-      w1[k] = rd[k] ? ld(a.cc + c.row[k]) : 0;
+      w1[k] = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
     }
     order_acquire();
 #pragma unroll
@@ -971,7 +982,7 @@ struct Silo {
         if (group_any<G>(conflict)) {
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            if (c.meta[k] & M_HELD) st(a.cc + c.row[k], silo::make(0, c.st[k]));
+            if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), silo::make(0, c.st[k]));
           abort_attempt(c, a, now);
         } else if (group_all<G>(!pending)) {
           validate = true;
@@ -1001,13 +1012,13 @@ struct Silo {
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
       if (!(m & M_VALID) || (m & M_WRITE)) continue;
-      const uint64_t w = ld(a.cc + c.row[k]);
+      const uint64_t w = ld(cc_word(a, c.row[k]));
       if (silo::ts(w) != c.st[k] || silo::locked(w)) fail = true;
     }
     if (group_any<G>(fail)) {
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if (c.meta[k] & M_HELD) st(a.cc + c.row[k], silo::make(0, c.st[k]));
+        if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), silo::make(0, c.st[k]));
       abort_attempt(c, a, now);
       return;
     }
@@ -1019,7 +1030,7 @@ struct Silo {
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], silo::make(0, commit_ts));
+      if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), silo::make(0, commit_ts));
     finish_tx(c, a, now, ticket, 0);
   }
 };
@@ -1092,7 +1103,7 @@ struct Tictoc {
         if (group_any<G>(conflict)) {
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+            if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), c.st[k]);
           abort_attempt(c, a, now);
         } else if (group_all<G>(!pending)) {
           // validation (tictoc.cuh:262-350)
@@ -1112,7 +1123,7 @@ struct Tictoc {
             const uint32_t m = c.meta[k];
             if (!(m & M_VALID) || (m & M_WRITE)) continue;
             if (tictoc::rts(c.st[k]) >= commit_ts) continue;
-            uint64_t* w = a.cc + c.row[k];
+            uint64_t* w = cc_word(a, c.row[k]);
             // first CAS from the snapshot word: if the word is unchanged it
             // succeeds without a load; otherwise it returns the current word
             uint64_t cur = c.st[k];
@@ -1134,7 +1145,7 @@ struct Tictoc {
           if (group_any<G>(fail)) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
-              if (c.meta[k] & M_HELD) st(a.cc + c.row[k], c.st[k]);
+              if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), c.st[k]);
             abort_attempt(c, a, now);
           } else {
             commit = true;
@@ -1154,7 +1165,7 @@ struct Tictoc {
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) st(a.cc + c.row[k], tictoc::make(0, 0, commit_ts));
+      if (c.meta[k] & M_HELD) st(cc_word(a, c.row[k]), tictoc::make(0, 0, commit_ts));
     finish_tx(c, a, now, commit_ts, ticket);
   }
 };
@@ -1189,7 +1200,7 @@ struct Ordered {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
-      seen[k] = ((m & M_VALID) && !(m & M_DONE)) ? ld32(a.counters + c.row[k]) : 0u;
+      seen[k] = ((m & M_VALID) && !(m & M_DONE)) ? ld32(turn_counter(a, c.row[k])) : 0u;
     }
     bool pending = false, any = false;
     bool ready[K];
@@ -1223,7 +1234,7 @@ struct Ordered {
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         if (!ready[k]) continue;
-        fetch_add32(a.counters + c.row[k], 1u);
+        fetch_add32(turn_counter(a, c.row[k]), 1u);
         c.meta[k] |= M_DONE;
       }
     }
@@ -1370,7 +1381,7 @@ __device__ void watchdog_stop(const TxCtx<G>& c, const ExecArgs& a) {
     for (int k = 0; k < K; ++k) {
       if (!(c.meta[k] & M_VALID)) continue;
       uint64_t* d = a.dbg + (c.op0 + op_index<G>(k)) * 4;
-      uint64_t* w = a.cc + c.row[k];
+      uint64_t* w = cc_word(a, c.row[k]);
       d[0] = ((uint64_t)c.meta[k] << 32) | c.row[k];
       d[1] = ld(w);
       d[2] = fetch_or(w, 0);

@@ -67,7 +67,7 @@ hipError_t inclusive_max_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint
 hipError_t access_group_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint32_t* flags,
                               hipStream_t s, uint32_t* rowstart = nullptr);
 // need[op] = gstart[p] + pos_base for op = sop[p] & OPID_MASK;
-// counters[srow[p]] = p + pos_base at row starts.
+// counters[row_slot(srow[p])] = p + pos_base at row starts.
 hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
                        uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
                        uint64_t nrows, hipStream_t s);
@@ -112,6 +112,8 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
                       double* ms_out);
 
 // --- per-batch state ---
+// clear the CC words (and MVCC chain heads) of rows[0, n) at their slots
+// (arrays of slot_words(nrows_table) words)
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
                       uint32_t* vprev, hipStream_t s);
 hipError_t batch_begin(uint64_t* ctrs, hipStream_t s);

@@ -47,6 +47,7 @@ constexpr uint64_t OFF_CTRS = OFF_ABORTS + al32(NA_MAX_TXS / 2);        // CTR_W
 constexpr uint64_t OFF_METRICS = OFF_CTRS + al32(CTR_WORDS);            // METRIC_SLOTS x Metrics
 constexpr uint64_t SCRATCH_WORDS = OFF_METRICS + al32((uint64_t)METRIC_SLOTS * METRICS_WORDS);
 static_assert(SCRATCH_WORDS * 8 == GCCTB_STATIC_SCRATCH_BYTES, "gcctb.h states the array's size");
+static_assert(slot_words(NA_MAX_ROWS) == NA_MAX_ROWS, "the rows' slots fit the CC and chain-head regions");
 static_assert(sizeof(Metrics) == METRICS_WORDS * 8, "metrics slot size");
 
 }  // namespace
@@ -183,9 +184,11 @@ extern "C" int gcctb_run_plain_static(const gcctb_static_run* r, gcctb_stats* ou
     uint64_t* metrics = base + OFF_METRICS;
     // The array keeps the last call's contents: clear what must start at
     // zero (fill kernels, which stream capture records as kernel nodes).
-    NA_TRY(launch::fill_u64(cc, r->rows, 0, s));
+    // per-row state sits at each row's slot (slot_words(rows) <= NA_MAX_ROWS)
+    const uint64_t words = slot_words(r->rows);
+    NA_TRY(launch::fill_u64(cc, words, 0, s));
     if (sch == SCHEME_MVCC) {
-      NA_TRY(launch::fill_u32(vprev, r->rows, 0, s));
+      NA_TRY(launch::fill_u32(vprev, words, 0, s));
       NA_TRY(launch::fill_u64(pool, 3 * NA_POOL_NODES, 0, s));
     }
     NA_TRY(launch::fill_u64(ctrs, CTR_WORDS, 0, s));
@@ -205,6 +208,7 @@ extern "C" int gcctb_run_plain_static(const gcctb_static_run* r, gcctb_stats* ou
     a.tx_off = r->tx_off;
     a.cc = cc;
     a.vprev = vprev;
+    a.slot_shift = slot_shift(r->rows);
     a.pool = pool;
     a.pool_cap = NA_POOL_NODES;
     a.ctrs = ctrs;
@@ -253,8 +257,8 @@ extern "C" int gcctb_run_plain_static(const gcctb_static_run* r, gcctb_stats* ou
     m = sum_metrics(metrics, s);
     if (m.error_flags) {
       // a stopped launch can leave locks or marks behind: clear them all
-      NA_TRY(launch::fill_u64(cc, r->rows, 0, s));
-      if (sch == SCHEME_MVCC) NA_TRY(launch::fill_u32(vprev, r->rows, 0, s));
+      NA_TRY(launch::fill_u64(cc, words, 0, s));
+      if (sch == SCHEME_MVCC) NA_TRY(launch::fill_u32(vprev, words, 0, s));
       NA_TRY(hipStreamSynchronize(s));
     }
   } catch (const NaFail& f) {

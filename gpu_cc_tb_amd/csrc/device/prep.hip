@@ -521,11 +521,13 @@ __global__ void k_access_group_flags(const uint32_t* srow, const uint32_t* sop, 
 __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart,
                               uint64_t n, uint32_t pos_base, uint32_t* need, uint32_t* counters,
                               uint64_t nrows) {
+  const uint32_t shift = slot_shift(nrows);
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
     need[sop[p] & OPID_MASK] = gstart[p] + pos_base;
     // an index miss probes to the index length; such ops never execute
-    if (((p == 0) || srow[p] != srow[p - 1]) && srow[p] < nrows) counters[srow[p]] = (uint32_t)p + pos_base;
+    if (((p == 0) || srow[p] != srow[p - 1]) && srow[p] < nrows)
+      counters[row_slot(srow[p], shift)] = (uint32_t)p + pos_base;
   }
 }
 
@@ -638,14 +640,15 @@ __global__ __launch_bounds__(256) void k_rank_hist(const uint32_t* rank, uint64_
       if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
-__global__ void k_reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
-                             uint32_t* vprev) {
+__global__ void k_reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint32_t shift,
+                             uint64_t* cc, uint32_t* vprev) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t r = rows[i];
     if (r >= nrows_table) continue;
-    if (cc) dev::st(cc + r, 0);
-    if (vprev) dev::st32(vprev + r, 0);
+    const uint32_t w = row_slot(r, shift);
+    if (cc) dev::st(cc + w, 0);
+    if (vprev) dev::st32(vprev + w, 0);
   }
 }
 __global__ void k_batch_begin(uint64_t* ctrs) {
@@ -790,13 +793,13 @@ hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, ui
   // scattered line per op: the cheaper form once n >= rows / 8.
   if (n * 8 >= nrows_table) {
     if (cc) {
-      const hipError_t e = fill_u64(cc, nrows_table, 0, s);
+      const hipError_t e = fill_u64(cc, slot_words(nrows_table), 0, s);
       if (e != hipSuccess) return e;
     }
-    if (vprev) return fill_u32(vprev, nrows_table, 0, s);
+    if (vprev) return fill_u32(vprev, slot_words(nrows_table), 0, s);
     return hipSuccess;
   }
-  GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, cc, vprev);
+  GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, slot_shift(nrows_table), cc, vprev);
 }
 hipError_t access_need_open(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart, const uint32_t* rstart,
                             uint64_t n, uint64_t nrows, const uint32_t* base, uint32_t* need, hipStream_t s) {

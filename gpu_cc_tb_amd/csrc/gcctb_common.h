@@ -238,6 +238,26 @@ GC_HD uint64_t idx_buckets_for(uint64_t n) { return n / 4 + 1; }
 // ---------------------------------------------------------------------------
 // Kernel argument block for the execution kernels.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Per-row CC state (CC words, MVCC chain heads, turn counters) sits at word
+// slot(row) = (row % SLOT_WAYS) << shift | row / SLOT_WAYS, where 2^shift is
+// the smallest power of two >= ceil(rows / SLOT_WAYS).  Zipf's hottest rows
+// are ids 0, 1, 2, ...; packed by row id, the 16 hottest CC words (32 turn
+// counters) share one 128-byte line, whose atomics then serialize and whose
+// pollers slow every handoff on it.  Spread this way, rows 0..15 sit on 16
+// lines 2^shift words apart.  The arrays hold SLOT_WAYS << shift words
+// (below 2 * rows + 16); a shift and an or keep the address math to two
+// 32-bit operations.
+// ---------------------------------------------------------------------------
+constexpr uint32_t SLOT_WAYS = 16;
+GC_HD constexpr uint32_t slot_shift(uint64_t rows) {
+  uint32_t s = 0;
+  while (((uint64_t)1 << s) * SLOT_WAYS < rows) ++s;
+  return s;
+}
+GC_HD constexpr uint64_t slot_words(uint64_t rows) { return (uint64_t)SLOT_WAYS << slot_shift(rows); }
+GC_HD constexpr uint32_t row_slot(uint32_t row, uint32_t shift) { return ((row % SLOT_WAYS) << shift) | (row / SLOT_WAYS); }
+
 struct ExecArgs {
   // table (row-major u64 cells)
   uint64_t* cells;
@@ -253,11 +273,12 @@ struct ExecArgs {
   uint64_t slot_lo;        // launch covers slots [slot_lo, slot_hi)
   uint64_t slot_hi;
   // CC state
-  uint64_t* cc;            // per-row CC word
-  uint32_t* vprev;         // MVCC: per-row head of the version chain (node+1)
+  uint64_t* cc;            // per-row CC word, at row_slot(row, slot_shift)
+  uint32_t* vprev;         // MVCC: per-row head of the version chain (node+1), at the row's slot
   uint64_t* pool;          // MVCC: version nodes, 3 u64 each
   uint64_t pool_cap;       // MVCC: node capacity
-  uint32_t* counters;      // gacco/gputx: per-row completed-access counter
+  uint32_t* counters;      // gacco/gputx: per-row completed-access counter, at the row's slot
+  uint32_t slot_shift;     // slot_shift(rows)
   const uint32_t* need;    // gacco/gputx: per-op access-group start
   uint64_t* ctrs;          // ticket, pool top and timestamp shards (CTR_* word offsets)
   // outputs

@@ -242,6 +242,7 @@ class SchemeHost {
     a.pool = ctx.pool.p;
     a.pool_cap = ctx.pool_cap;
     a.counters = ctx.counters.p;
+    a.slot_shift = slot_shift(ctx.cfg.rows);
     a.need = ctx.need.p;
     a.ctrs = ctx.ctrs.p;
     a.rows_out = ctx.rows_out.p;
@@ -580,24 +581,24 @@ void enqueue_error_copy(gcctb_ctx& c) {
 // restore after a clean batch.  Every CC word, chain head and turn counter is
 // cleared, which is the state outside a run.
 void clear_cc_state(gcctb_ctx& c) {
-  const uint64_t rows = c.cfg.rows;
-  HIP_TRY(launch::fill_u64(c.cc.p, rows, 0, c.stream));
-  if (c.vprev.p) HIP_TRY(launch::fill_u32(c.vprev.p, rows, 0, c.stream));
-  if (c.counters.p) HIP_TRY(launch::fill_u32(c.counters.p, rows, 0, c.stream));
+  HIP_TRY(launch::fill_u64(c.cc.p, c.cc.n, 0, c.stream));
+  if (c.vprev.p) HIP_TRY(launch::fill_u32(c.vprev.p, c.vprev.n, 0, c.stream));
+  if (c.counters.p) HIP_TRY(launch::fill_u32(c.counters.p, c.counters.n, 0, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
 }
 
 void ensure_state(gcctb_ctx& c) {
   const uint64_t rows = c.cfg.rows;
-  c.cc.alloc(rows);
-  HIP_TRY(hipMemsetAsync(c.cc.p, 0, rows * 8, c.stream));
+  const uint64_t words = slot_words(rows);  // per-row state at each row's slot
+  c.cc.alloc(words);
+  HIP_TRY(hipMemsetAsync(c.cc.p, 0, words * 8, c.stream));
   if (c.scheme == SCHEME_MVCC) {
-    c.vprev.alloc(rows);
-    HIP_TRY(hipMemsetAsync(c.vprev.p, 0, rows * 4, c.stream));
+    c.vprev.alloc(words);
+    HIP_TRY(hipMemsetAsync(c.vprev.p, 0, words * 4, c.stream));
   }
   if (c.scheme == SCHEME_GACCO || c.scheme == SCHEME_GPUTX) {
-    c.counters.alloc(rows);
-    HIP_TRY(hipMemsetAsync(c.counters.p, 0, rows * 4, c.stream));
+    c.counters.alloc(words);
+    HIP_TRY(hipMemsetAsync(c.counters.p, 0, words * 4, c.stream));
   }
   c.ctrs.alloc(CTR_WORDS);
   HIP_TRY(hipMemsetAsync(c.ctrs.p, 0, CTR_WORDS * sizeof(uint64_t), c.stream));
@@ -1495,7 +1496,11 @@ int gcctb_fetch_cc(gcctb_ctx* c, uint64_t* words) {
   if (!c || !words) return set_error(GCCTB_EINVAL, "null argument");
   try {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(words, c->cc.p, c->cfg.rows * 8, hipMemcpyDeviceToHost));
+    // the words sit at their rows' slots: gather them back into row order
+    std::vector<uint64_t> slots(c->cc.n);
+    HIP_TRY(hipMemcpy(slots.data(), c->cc.p, c->cc.n * 8, hipMemcpyDeviceToHost));
+    const uint32_t shift = slot_shift(c->cfg.rows);
+    for (uint64_t r = 0; r < c->cfg.rows; ++r) words[r] = slots[row_slot((uint32_t)r, shift)];
   } catch (const HipFailure& f) {
     return hip_fail(f);
   }
