@@ -17,6 +17,8 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libgcctb.so")
 
 SCHEMES = ("tpl_nw", "tpl_wd", "to", "mvcc", "silo", "tictoc", "gacco", "gputx")
+# the latch schemes (benchmark_cpu/ycsb.cu:170-190): plain and graph forms, column mode
+LATCH_SCHEMES = ("slow_tpl_nw", "slow_tpl_wd", "slow_to", "slow_mvcc", "slow_silo", "slow_tictoc")
 SEED = 141919810  # the reference's SEED (benchmark_cpu/ycsb.cu:27, generation/gen_ycsb.cu:12)
 
 GCCTB_OK = 0

@@ -39,6 +39,7 @@ DEVICE_SRCS = [
     "device/exec_row.hip",
     "device/exec_evt.hip",
     "device/exec_row_evt.hip",
+    "device/exec_latch.hip",
     "device/microbench.hip",
     "device/noalloc.hip",
 ]

@@ -21,21 +21,30 @@
 #pragma once
 #include "dev_util.h"
 
-// Two variants are separate sets of kernels, so that the column-mode kernels
-// of namespace dev carry none of their code: whole-row mode (every op moves
-// its whole row, the reference's Item copies, benchmark/ycsb.cu:57,86-91) and
-// the debug event log (the reference's AddEvent, include/runtime.cuh:64-81).
-// A translation unit that defines GCCTB_ROW_OPS and/or GCCTB_EVENTS to 1
-// compiles this file into namespace dev_row, dev_evt or dev_row_evt.
+// Three variants are separate sets of kernels, so that the column-mode
+// kernels of namespace dev carry none of their code: whole-row mode (every op
+// moves its whole row, the reference's Item copies, benchmark/ycsb.cu:57,
+// 86-91), the debug event log (the reference's AddEvent,
+// include/runtime.cuh:64-81) and the latch schemes (the reference's slow_*
+// schemes, include/slow_twopl.cuh etc.: every access to a row's CC word is
+// made under the row's latch).  A translation unit that defines
+// GCCTB_ROW_OPS and/or GCCTB_EVENTS, or GCCTB_LATCH, to 1 compiles this file
+// into namespace dev_row, dev_evt, dev_row_evt or dev_latch.
 #ifndef GCCTB_ROW_OPS
 #define GCCTB_ROW_OPS 0
 #endif
 #ifndef GCCTB_EVENTS
 #define GCCTB_EVENTS 0
 #endif
+#ifndef GCCTB_LATCH
+#define GCCTB_LATCH 0
+#endif
 
 namespace gcctb {
-#if GCCTB_ROW_OPS && GCCTB_EVENTS
+#if GCCTB_LATCH
+namespace dev_latch {
+using namespace dev;
+#elif GCCTB_ROW_OPS && GCCTB_EVENTS
 namespace dev_row_evt {
 using namespace dev;
 #elif GCCTB_ROW_OPS
@@ -50,6 +59,8 @@ namespace dev {
 
 constexpr bool kRowOps = GCCTB_ROW_OPS != 0;
 constexpr bool kEvents = GCCTB_EVENTS != 0;
+constexpr bool kLatch = GCCTB_LATCH != 0;
+static_assert(!(kLatch && (kRowOps || kEvents)), "the latch schemes run in column mode without the event log");
 
 // per-op slot flags
 constexpr uint32_t M_VALID = 1u;   // op exists and its key was found in the index
@@ -88,9 +99,60 @@ struct TxCtx {
   uint64_t st[K];
 };
 
-// A row's CC word, MVCC chain head and turn counter (at its slot).
+// A row's CC word, MVCC chain head and turn counter (at its slot).  The
+// latch schemes give each slot 16 bytes: the CC word, then the row's latch.
 __device__ __forceinline__ uint64_t* cc_word(const ExecArgs& a, uint32_t row) {
-  return a.cc + row_slot(row, a.slot_shift);
+  return a.cc + ((uint64_t)row_slot(row, a.slot_shift) << (kLatch ? 1 : 0));
+}
+
+// Accesses to a CC word.  In the latch schemes each read or read-modify-write
+// runs under the row's latch, taken with a CAS and dropped with a store, as
+// the reference's slow_* schemes guard their metadata (common::latch_lock /
+// latch_unlock, include/runtime.cuh:91-112).  The access inside stays the same
+// atomic one, so correctness never rests on the latch: a lane that fails
+// LATCH_TRIES times to take it makes the access without it, and no lane ever
+// waits on a latch beyond that (a lane spinning on a latch held by another
+// lane of its own wave would wait forever).  Each attempt tests the latch
+// with a load before its CAS and pauses 512 cycles after a miss.  Plain
+// stores (unlock, publish) go without the latch, as in the reference.
+constexpr int LATCH_TRIES = 16;
+__device__ __forceinline__ uint32_t* latch_of(uint64_t* w) { return reinterpret_cast<uint32_t*>(w + 1); }
+template <class F>
+__device__ __forceinline__ auto latched(uint64_t* w, F f) {
+  if constexpr (!kLatch) {
+    return f();
+  } else {
+    uint32_t* l = latch_of(w);
+    bool held = false;
+    // test and test-and-set: a CAS only when the latch reads free, and a
+    // pause after a miss, so that the waiters of a hot row do not flood its
+    // line with failing CASes that the holder's release queues behind
+#pragma unroll 1
+    for (int t = 0; t < LATCH_TRIES && !held; ++t) {
+      uint32_t free_latch = 0;
+      held = ld32(l) == 0u && cas32(l, free_latch, 1u);
+      if (!held) __builtin_amdgcn_s_sleep(8);
+    }
+    if (held) order_acquire();
+    const auto r = f();
+    if (held) {
+      order_release();
+      st32(l, 0u);
+    }
+    return r;
+  }
+}
+__device__ __forceinline__ uint64_t w_ld(uint64_t* w) {
+  return latched(w, [&] { return ld(w); });
+}
+__device__ __forceinline__ bool w_cas(uint64_t* w, uint64_t& cur, uint64_t nxt) {
+  return latched(w, [&] { return cas(w, cur, nxt); });
+}
+__device__ __forceinline__ uint64_t w_or(uint64_t* w, uint64_t bits) {
+  return latched(w, [&] { return fetch_or(w, bits); });
+}
+__device__ __forceinline__ uint64_t w_sub(uint64_t* w, uint64_t x) {
+  return latched(w, [&] { return fetch_sub(w, x); });
 }
 __device__ __forceinline__ uint32_t* chain_head(const ExecArgs& a, uint32_t row) {
   return a.vprev + row_slot(row, a.slot_shift);
@@ -285,7 +347,7 @@ struct TwoPL {
         if (h < self) return R_ABORT;  // die: an older transaction holds it
         return R_WAIT;                 // wait for a younger holder
       }
-      if (cas(w, cur, cur + tpl::CNT_ONE)) return R_OK;
+      if (w_cas(w, cur, cur + tpl::CNT_ONE)) return R_OK;
     }
     return R_WAIT;
   }
@@ -295,7 +357,7 @@ struct TwoPL {
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
       if (cur == 0) {
-        if (cas(w, cur, tpl::make_excl(self))) return R_OK;
+        if (w_cas(w, cur, tpl::make_excl(self))) return R_OK;
         continue;
       }
       if (!tpl::excl(cur)) return R_ABORT;
@@ -313,7 +375,7 @@ struct TwoPL {
     if (excl)
       st(w, 0);
     else
-      fetch_sub(w, tpl::CNT_ONE);
+      w_sub(w, tpl::CNT_ONE);
   }
 
   template <int G>
@@ -339,7 +401,7 @@ struct TwoPL {
           cur[k] = 0;
           got[k] = false;
           if ((m & M_VALID) && !(m & M_HELD))
-            got[k] = cas(cc_word(a, c.row[k]), cur[k], (m & M_WRITE) ? tpl::make_excl(c.tx) : tpl::CNT_ONE);
+            got[k] = w_cas(cc_word(a, c.row[k]), cur[k], (m & M_WRITE) ? tpl::make_excl(c.tx) : tpl::CNT_ONE);
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -430,11 +492,11 @@ __device__ __forceinline__ bool hold_version(uint64_t* w, uint64_t ts, uint64_t 
     if (fresh && tsw::version(cur) != ver) return false;
     if (tsw::rts(cur) >= ts) {
       if (fresh) return true;
-      cur = ld(w);
+      cur = w_ld(w);
       fresh = true;
       continue;
     }
-    if (cas(w, cur, tsw::make(0, ts, tsw::wts(cur)))) return true;
+    if (w_cas(w, cur, tsw::make(0, ts, tsw::wts(cur)))) return true;
     fresh = true;
   }
   return false;
@@ -458,7 +520,7 @@ __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs&
   for (int k = 0; k < K; ++k) {
     const uint32_t m = c.meta[k];
     rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
-    cur[k] = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
+    cur[k] = rd[k] ? w_ld(cc_word(a, c.row[k])) : 0;
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -489,9 +551,9 @@ __device__ __forceinline__ void ts_head_reads(const TxCtx<G>& c, const ExecArgs&
     ok[k] = false;
     if (!cand[k]) continue;
     if (tsw::rts(cur[k]) >= ts)
-      cur[k] = ld(cc_word(a, c.row[k]));  // nothing to raise: re-check the version
+      cur[k] = w_ld(cc_word(a, c.row[k]));  // nothing to raise: re-check the version
     else
-      ok[k] = cas(cc_word(a, c.row[k]), cur[k], tsw::make(0, ts, tsw::wts(cur[k])));
+      ok[k] = w_cas(cc_word(a, c.row[k]), cur[k], tsw::make(0, ts, tsw::wts(cur[k])));
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -505,7 +567,7 @@ struct TimestampOrdering {
   static constexpr bool kRestInOps = false;  // t_resume paces restarts only (PH_START)
   static constexpr bool kBulkRefill = true;
   static __device__ StepResult write_mark(uint64_t* w, uint64_t ts, uint64_t& prev, uint64_t& seen) {
-    uint64_t cur = ld(w);
+    uint64_t cur = w_ld(w);
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
       if (tsw::rts(cur) > ts || tsw::wts(cur) > ts || tsw::unc(cur)) {
@@ -513,7 +575,7 @@ struct TimestampOrdering {
         return R_ABORT;
       }
       prev = cur;
-      if (cas(w, cur, tsw::make(1, ts, ts))) return R_OK;
+      if (w_cas(w, cur, tsw::make(1, ts, ts))) return R_OK;
     }
     return R_WAIT;
   }
@@ -632,7 +694,7 @@ struct Mvcc {
     uint64_t* w = cc_word(a, row);
     const uint32_t meta = col << M_COL_SHIFT;
     const uint64_t stride = mvcc_stride(a);
-    uint64_t cur = ld(w);
+    uint64_t cur = w_ld(w);
 #pragma unroll 1
     for (int t = 0; t < CAS_TRIES; ++t) {
       if (tsw::unc(cur)) return R_WAIT;
@@ -653,7 +715,7 @@ struct Mvcc {
       uint32_t p = ld32(chain_head(a, row));
       uint64_t v = op_load(a, row, meta);
       order_acquire();
-      const uint64_t after = ld(w);
+      const uint64_t after = w_ld(w);
       if (tsw::version(after) != tsw::version(cur)) {
         cur = after;
         continue;
@@ -747,7 +809,7 @@ struct Mvcc {
         const uint32_t m = c.meta[k];
         if (!(m & M_VALID) || !(m & M_WRITE) || (m & M_DONE)) continue;
         uint64_t* w = cc_word(a, c.row[k]);
-        uint64_t cur = ld(w);
+        uint64_t cur = w_ld(w);
         StepResult r = R_WAIT;
 #pragma unroll 1
         for (int t = 0; t < CAS_TRIES; ++t) {
@@ -758,7 +820,7 @@ struct Mvcc {
             break;
           }
           c.st[k] = cur;
-          if (cas(w, cur, tsw::make(1, c.ts, c.ts))) {
+          if (w_cas(w, cur, tsw::make(1, c.ts, c.ts))) {
             r = R_OK;
             break;
           }
@@ -863,7 +925,7 @@ __device__ __forceinline__ void occ_snapshot(const TxCtx<G>& c, const ExecArgs& 
   for (int k = 0; k < K; ++k) {
     const uint32_t m = c.meta[k];
     rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
-    w1[k] = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
+    w1[k] = rd[k] ? w_ld(cc_word(a, c.row[k])) : 0;
   }
   order_acquire();
 #pragma unroll
@@ -871,7 +933,7 @@ __device__ __forceinline__ void occ_snapshot(const TxCtx<G>& c, const ExecArgs& 
   order_acquire();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const uint64_t w2 = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
+    const uint64_t w2 = rd[k] ? w_ld(cc_word(a, c.row[k])) : 0;
     ok[k] = rd[k] && w1[k] == w2 && !(w1[k] & 1u);
   }
 }
@@ -887,7 +949,7 @@ __device__ __forceinline__ void occ_lock(const TxCtx<G>& c, const ExecArgs& a, u
   for (int k = 0; k < K; ++k) {
     const uint32_t m = c.meta[k];
     tried[k] = (m & M_VALID) && (m & M_WRITE) && !(m & M_HELD);
-    old[k] = tried[k] ? fetch_or(cc_word(a, c.row[k]), 1ull) : 0;
+    old[k] = tried[k] ? w_or(cc_word(a, c.row[k]), 1ull) : 0;
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) got[k] = tried[k] && !(old[k] & 1u);
@@ -920,7 +982,7 @@ struct Silo {
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
       rd[k] = (m & M_VALID) && !(m & (M_DONE | M_WRITE));
-      w1[k] = rd[k] ? ld(cc_word(a, c.row[k])) : 0;
+      w1[k] = rd[k] ? w_ld(cc_word(a, c.row[k])) : 0;
     }
     order_acquire();
 #pragma unroll
@@ -1012,7 +1074,7 @@ struct Silo {
     for (int k = 0; k < K; ++k) {
       const uint32_t m = c.meta[k];
       if (!(m & M_VALID) || (m & M_WRITE)) continue;
-      const uint64_t w = ld(cc_word(a, c.row[k]));
+      const uint64_t w = w_ld(cc_word(a, c.row[k]));
       if (silo::ts(w) != c.st[k] || silo::locked(w)) fail = true;
     }
     if (group_any<G>(fail)) {
@@ -1139,7 +1201,7 @@ struct Tictoc {
               const uint64_t delta = commit_ts - wts;
               const uint64_t shift = delta - (delta & 0x7fffull);
               const uint64_t nxt = tictoc::make(tictoc::locked(cur), delta - shift, wts + shift);
-              if (cas(w, cur, nxt)) break;
+              if (w_cas(w, cur, nxt)) break;
             }
           }
           if (group_any<G>(fail)) {

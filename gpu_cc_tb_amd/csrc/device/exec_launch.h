@@ -6,7 +6,9 @@
 #include "launch.h"
 
 namespace gcctb {
-#if GCCTB_ROW_OPS && GCCTB_EVENTS
+#if GCCTB_LATCH
+namespace dev_latch {
+#elif GCCTB_ROW_OPS && GCCTB_EVENTS
 namespace dev_row_evt {
 #elif GCCTB_ROW_OPS
 namespace dev_row {
@@ -56,10 +58,11 @@ hipError_t launch_exec_g(int G, const ExecArgs& args, uint64_t nslots, int threa
   a.persist = persist ? 1u : 0u;
   // persistent launches of at most 256 threads per block take the
   // instantiation bounded by that block size (more registers, no spills);
-  // whole-row mode and the event log have the plain (per-batch) kernels only
+  // whole-row mode, the event log and the latch schemes have the plain
+  // (per-batch) kernels only
 #define GC_CASE(g)                                                                      \
   case g:                                                                               \
-    if constexpr (kRowOps || kEvents) {                                                 \
+    if constexpr (kRowOps || kEvents || kLatch) {                                       \
       return persist ? hipErrorInvalidValue : launch_one<S, g, false, 1024>(a, nslots, threads, s); \
     } else {                                                                            \
       return persist ? (threads <= 256 ? launch_one<S, g, true, 256>(a, nslots, threads, s)    \
@@ -78,5 +81,5 @@ hipError_t launch_exec_g(int G, const ExecArgs& args, uint64_t nslots, int threa
 #undef GC_CASE
 }
 
-}  // namespace dev (dev_row, dev_evt, dev_row_evt)
+}  // namespace dev (dev_row, dev_evt, dev_row_evt, dev_latch)
 }  // namespace gcctb

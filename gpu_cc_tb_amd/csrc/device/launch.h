@@ -21,6 +21,9 @@ hipError_t exec_ordered(int G, const ExecArgs& a, uint64_t nslots, int threads, 
 hipError_t exec_row(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
 hipError_t exec_evt(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
 hipError_t exec_row_evt(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+// the latch schemes (slow_*), plain form, column mode: the CC arrays hold a
+// 16-byte slot per row (the word, then the latch)
+hipError_t exec_latch(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
 
 // --- table / index ---
 hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s);
@@ -113,9 +116,10 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
 
 // --- per-batch state ---
 // clear the CC words (and MVCC chain heads) of rows[0, n) at their slots
-// (arrays of slot_words(nrows_table) words)
+// (arrays of slot_words(nrows_table) words); latch: the CC array holds two
+// words per slot (the word and the row's latch)
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
-                      uint32_t* vprev, hipStream_t s);
+                      uint32_t* vprev, hipStream_t s, bool latch = false);
 hipError_t batch_begin(uint64_t* ctrs, hipStream_t s);
 // admission state of a persistent launch (slot counter, gates, admission words)
 hipError_t persist_begin(uint64_t* ctrs, hipStream_t s);

@@ -641,13 +641,13 @@ __global__ __launch_bounds__(256) void k_rank_hist(const uint32_t* rank, uint64_
 }
 
 __global__ void k_reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint32_t shift,
-                             uint64_t* cc, uint32_t* vprev) {
+                             uint32_t wshift, uint64_t* cc, uint32_t* vprev) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t r = rows[i];
     if (r >= nrows_table) continue;
     const uint32_t w = row_slot(r, shift);
-    if (cc) dev::st(cc + w, 0);
+    if (cc) dev::st(cc + ((uint64_t)w << wshift), 0);
     if (vprev) dev::st32(vprev + w, 0);
   }
 }
@@ -787,19 +787,21 @@ hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks, uint32
   return hipGetLastError();
 }
 hipError_t reset_rows(const uint32_t* rows, uint64_t n, uint64_t nrows_table, uint64_t* cc,
-                      uint32_t* vprev, hipStream_t s) {
+                      uint32_t* vprev, hipStream_t s, bool latch) {
   // Outside a run every CC word and chain head is zero, so clearing the whole
   // arrays leaves the same state.  That streams rows x 12 bytes instead of one
-  // scattered line per op: the cheaper form once n >= rows / 8.
+  // scattered line per op: the cheaper form once n >= rows / 8.  (A latch
+  // array is zero outside a critical section, so its fill changes nothing.)
+  const uint32_t wshift = latch ? 1 : 0;
   if (n * 8 >= nrows_table) {
     if (cc) {
-      const hipError_t e = fill_u64(cc, slot_words(nrows_table), 0, s);
+      const hipError_t e = fill_u64(cc, slot_words(nrows_table) << wshift, 0, s);
       if (e != hipSuccess) return e;
     }
     if (vprev) return fill_u32(vprev, slot_words(nrows_table), 0, s);
     return hipSuccess;
   }
-  GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, slot_shift(nrows_table), cc, vprev);
+  GC_LAUNCH(k_reset_rows, n, rows, n, nrows_table, slot_shift(nrows_table), wshift, cc, vprev);
 }
 hipError_t access_need_open(const uint32_t* srow, const uint32_t* sop, const uint32_t* gstart, const uint32_t* rstart,
                             uint64_t n, uint64_t nrows, const uint32_t* base, uint32_t* need, hipStream_t s) {

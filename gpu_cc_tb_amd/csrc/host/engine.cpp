@@ -73,12 +73,22 @@ int scheme_from_name(const char* s) {
   return -1;
 }
 
+// The latch schemes slow_tpl_nw, slow_tpl_wd, slow_to, slow_mvcc, slow_silo
+// and slow_tictoc (benchmark_cpu/ycsb.cu:170-190): the id of the scheme they
+// guard with latches, or -1.
+int latch_scheme_from_name(const char* s) {
+  if (!s || std::strncmp(s, "slow_", 5) != 0) return -1;
+  const int base = scheme_from_name(s + 5);
+  return base >= 0 && base != SCHEME_GACCO && base != SCHEME_GPUTX ? base : -1;
+}
+
 class SchemeHost;
 
 }  // namespace gcctb
 
 struct gcctb_ctx {
   gcctb::Scheme scheme;
+  bool latch = false;  // a slow_* scheme: CC-word accesses under per-row latches
   gcctb_config cfg;
   int G = 1;
   int threads = 1024;
@@ -89,7 +99,7 @@ struct gcctb_ctx {
   // scheme's measured default (-1; GCCTB_PROBE_PASS sets 0 or 1)
   int probe_pass = -1;
   uint32_t backoff_exp = 10;
-  uint32_t poll_ticks = 25;  // 0.25 us (measured sweep 6..100 at theta 0.99)
+  uint32_t poll_ticks = 10;  // 0.1 us (measured at theta 0.99 with the hot rows' turn counters spread)
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   // pinned host word: the error flags of the launches since the last metrics
@@ -218,7 +228,7 @@ class SchemeHost {
     (void)batch_id;
     const uint64_t o0 = ctx.h_tx_off[batch_st], o1 = ctx.h_tx_off[batch_st + n];
     HIP_TRY(launch::reset_rows(ctx.rows_out.p + o0, o1 - o0, ctx.cfg.rows, ctx.cc.p,
-                               ctx.scheme == SCHEME_MVCC ? ctx.vprev.p : nullptr, ctx.stream));
+                               ctx.scheme == SCHEME_MVCC ? ctx.vprev.p : nullptr, ctx.stream, ctx.latch));
   }
   // Device allocations of the scheme (GetMemSize, include/cc.cuh:39).
   virtual uint64_t GetMemSize() const { return ctx.cc.n * 8; }
@@ -272,10 +282,13 @@ class SchemeHost {
   // Kernel variants (whole-row mode, the event log): every scheme's plain
   // kernels of a variant come from one launcher.
   bool launch_rows(const ExecArgs& a, uint64_t n, bool persist) {
-    if (!ctx.cfg.row_ops && !ctx.cfg.debug_events) return false;
-    if (persist) throw HipFailure{hipErrorInvalidValue, "whole-row mode and the event log have no persistent kernels"};
+    if (!ctx.cfg.row_ops && !ctx.cfg.debug_events && !ctx.latch) return false;
+    if (persist)
+      throw HipFailure{hipErrorInvalidValue, "whole-row mode, the event log and the latch schemes have no persistent kernels"};
     const int sch = (int)ctx.scheme;
-    if (ctx.cfg.row_ops && ctx.cfg.debug_events)
+    if (ctx.latch)
+      HIP_TRY(launch::exec_latch(sch, ctx.G, a, n, ctx.threads, ctx.stream));
+    else if (ctx.cfg.row_ops && ctx.cfg.debug_events)
       HIP_TRY(launch::exec_row_evt(sch, ctx.G, a, n, ctx.threads, ctx.stream));
     else if (ctx.cfg.row_ops)
       HIP_TRY(launch::exec_row(sch, ctx.G, a, n, ctx.threads, ctx.stream));
@@ -590,8 +603,10 @@ void clear_cc_state(gcctb_ctx& c) {
 void ensure_state(gcctb_ctx& c) {
   const uint64_t rows = c.cfg.rows;
   const uint64_t words = slot_words(rows);  // per-row state at each row's slot
-  c.cc.alloc(words);
-  HIP_TRY(hipMemsetAsync(c.cc.p, 0, words * 8, c.stream));
+  // the latch schemes keep the row's latch in the word after its CC word
+  const uint64_t cc_words = c.latch ? 2 * words : words;
+  c.cc.alloc(cc_words);
+  HIP_TRY(hipMemsetAsync(c.cc.p, 0, cc_words * 8, c.stream));
   if (c.scheme == SCHEME_MVCC) {
     c.vprev.alloc(words);
     HIP_TRY(hipMemsetAsync(c.vprev.p, 0, words * 4, c.stream));
@@ -617,8 +632,11 @@ const char* gcctb_last_error(void) { return g_err.c_str(); }
 int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   if (!cfg || !out) return set_error(GCCTB_EINVAL, "null argument");
   *out = nullptr;
-  const int sch = scheme_from_name(cfg->cc_scheme);
+  const int latch_base = latch_scheme_from_name(cfg->cc_scheme);
+  const int sch = latch_base >= 0 ? latch_base : scheme_from_name(cfg->cc_scheme);
   if (sch < 0) return set_error(GCCTB_EINVAL, std::string("unknown cc_scheme ") + (cfg->cc_scheme ? cfg->cc_scheme : "(null)"));
+  if (latch_base >= 0 && (cfg->row_ops || cfg->debug_events))
+    return set_error(GCCTB_EINVAL, "the latch schemes run in column mode without the event log");
   // benchmark_cpu/ycsb.cu:97-98: warp_cnt > 32 || valid_txn_bitoffset > 5 -> -1
   if (cfg->block_size < 1 || cfg->block_size > 32 || cfg->warp_density < 0 || cfg->warp_density > 5)
     return set_error(GCCTB_EINVAL, "warp_density must be 0..5 and block_size 1..32");
@@ -639,6 +657,7 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   auto* c = new (std::nothrow) gcctb_ctx();
   if (!c) return set_error(GCCTB_ENOMEM, "out of host memory");
   c->scheme = (Scheme)sch;
+  c->latch = latch_base >= 0;
   c->cfg = *cfg;
   c->cfg.cc_scheme = nullptr;
   c->G = 1 << cfg->warp_density;
@@ -658,7 +677,7 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
       c->backoff_exp = (uint32_t)(e < 0 ? 0 : e > 16 ? 16 : e);
     }
     // GaccO / GPUTx: assumed handoff time per turn on a row, in 10 ns ticks,
-    // that paces a waiter's rest (GCCTB_POLL_TICKS, 1..1000, default 25)
+    // that paces a waiter's rest (GCCTB_POLL_TICKS, 1..1000, default 10)
     if (const char* pt = std::getenv("GCCTB_POLL_TICKS")) {
       const int t = std::atoi(pt);
       c->poll_ticks = (uint32_t)(t < 1 ? 1 : t > 1000 ? 1000 : t);
@@ -965,8 +984,8 @@ static std::string error_text(uint64_t flags) {
 static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
   if (!c) return set_error(GCCTB_EINVAL, "null context");
   if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
-  if (form == FORM_PERSIST && (c->cfg.row_ops || c->cfg.debug_events))
-    return set_error(GCCTB_EINVAL, "whole-row mode and the event log run the plain and graph forms");
+  if (form == FORM_PERSIST && (c->cfg.row_ops || c->cfg.debug_events || c->latch))
+    return set_error(GCCTB_EINVAL, "whole-row mode, the event log and the latch schemes run the plain and graph forms");
   gcctb_stats st{};
   Metrics m{};
   try {
@@ -1191,8 +1210,8 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
   if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
   if (c->scheme == SCHEME_GPUTX)
     return set_error(GCCTB_EINVAL, "gputx ranks its whole input before running; it has no serving form");
-  if (c->cfg.row_ops || c->cfg.debug_events)
-    return set_error(GCCTB_EINVAL, "whole-row mode and the event log run the plain and graph forms");
+  if (c->cfg.row_ops || c->cfg.debug_events || c->latch)
+    return set_error(GCCTB_EINVAL, "whole-row mode, the event log and the latch schemes run the plain and graph forms");
   uint64_t total = 0;
   for (uint64_t i = 0; i < nbatches; ++i) {
     if (batch_sizes[i] == 0) return set_error(GCCTB_EINVAL, "empty batch");
@@ -1500,7 +1519,8 @@ int gcctb_fetch_cc(gcctb_ctx* c, uint64_t* words) {
     std::vector<uint64_t> slots(c->cc.n);
     HIP_TRY(hipMemcpy(slots.data(), c->cc.p, c->cc.n * 8, hipMemcpyDeviceToHost));
     const uint32_t shift = slot_shift(c->cfg.rows);
-    for (uint64_t r = 0; r < c->cfg.rows; ++r) words[r] = slots[row_slot((uint32_t)r, shift)];
+    const uint32_t wshift = c->latch ? 1 : 0;
+    for (uint64_t r = 0; r < c->cfg.rows; ++r) words[r] = slots[(uint64_t)row_slot((uint32_t)r, shift) << wshift];
   } catch (const HipFailure& f) {
     return hip_fail(f);
   }

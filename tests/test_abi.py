@@ -33,7 +33,7 @@ def test_bad_config_is_rejected_before_device_use():
             g.Context("tpl_nw", rows=16, **kw)
         assert e.value.code == -1
     with pytest.raises(g.GcctbError) as e:
-        g.Context("slow_silo", rows=16)
+        g.Context("no_such_scheme", rows=16)
     assert e.value.code == -1
 
 
@@ -225,3 +225,17 @@ def test_txs_reader_hand_packed_records():
     assert g.op_key(r.ops).tolist() == keys
     assert g.op_is_write(r.ops).tolist() == writes
     assert np.all(g.op_col(r.ops) < 13)
+
+
+def test_latch_scheme_names_checked_before_device_use():
+    """slow_* names are the reference's latch schemes (benchmark_cpu/ycsb.cu:
+    170-190); there is no slow_gacco / slow_gputx, and the latch schemes run
+    in column mode without the event log."""
+    for name in ["slow_gacco", "slow_gputx", "slow_", "slow_slow_silo"]:
+        with pytest.raises(g.GcctbError) as e:
+            g.Context(name, rows=16)
+        assert e.value.code == -1, name
+    for kw in [dict(row_ops=True), dict(debug_events=True)]:
+        with pytest.raises(g.GcctbError) as e:
+            g.Context("slow_silo", rows=16, **kw)
+        assert e.value.code == -1
