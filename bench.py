@@ -23,7 +23,9 @@ the timed region, at full size):
            ops, per-batch theta U[0, 0.99], read ratio 0.5 / 0.9 / 1.0) on
            10,485,760 rows: persistent form against per-batch plain launches.
   theta99: 1,048,576 transactions of 16 ops at Zipf theta 0.99, read ratio 0.9
-           and 0.5, batch 65,536, on 10,485,760 rows: persistent over plain.
+           and 0.5, on 10,485,760 rows: persistent over plain at the same
+           launch shape and batch 65,536, and over plain at its own tuned
+           shape and batch size.
 
 Latency: the serving form on a 67,108,864-row table with requests of 16 ops,
 theta 0.9, read ratio 0.8, in batches of log-uniform size in [256, 65,536]
@@ -211,23 +213,34 @@ def stream_leg(args, table, schemes):
 
 
 def theta99_leg(args, table, schemes):
-    """Theta 0.99, 1,048,576 tx x 16 ops, batch 65,536: persistent over plain
-    at the tuned theta-0.99 shape, per scheme, at read ratio 0.9 and 0.5."""
+    """Theta 0.99, 1,048,576 tx x 16 ops: persistent at the tuned theta-0.99
+    shape against the plain form twice, per scheme, at read ratio 0.9 and 0.5:
+    at the same shape with batch 65,536 (the reference's per-batch structure
+    on the same launch shape), and at the plain form's own tuned shape and
+    batch size (best against best)."""
     out = {}
     for rr in (0.9, 0.5):
         w = g.generate(args.rows, args.txs, theta=0.99, read_ratio=rr, len_min=16)
         res = {}
         for sc in schemes:
             shape = autotune.lookup(sc, 0.99, rr, table)
+            pshape = autotune.lookup_plain(sc, 0.99, rr, table)
             p = _one_pass(sc, args.rows, w, shape, "persistent")
             q = _one_pass(sc, args.rows, w, shape, "plain")
-            pv, qv = w.ntx / (p["tot_ms"] / 1e3), w.ntx / (q["tot_ms"] / 1e3)
+            t = _one_pass(sc, args.rows, w, dict(pshape, window=0), "plain", batch=pshape["batch_size"])
+            pv, qv, tv = (w.ntx / (x["tot_ms"] / 1e3) for x in (p, q, t))
             res[sc] = {"persistent_txps": round(pv), "plain_txps": round(qv), "ratio": round(pv / qv, 3),
+                       "plain_tuned_txps": round(tv), "ratio_vs_tuned_plain": round(pv / tv, 3),
                        "persistent_aborts": p["aborts"], "plain_aborts": q["aborts"],
-                       "wd": shape["warp_density"], "bs": shape["block_size"], "window": shape["window"]}
-            _progress(f"theta99 r{rr} {sc}: persistent {pv / 1e6:.2f} M, plain {qv / 1e6:.2f} M Tx/s ({pv / qv:.2f}x)")
-        out[f"read_{rr}"] = {"per_scheme": res, "min_ratio": min(v["ratio"] for v in res.values()) if res else None}
-    return {"workload": f"{args.rows} rows, {args.txs} tx x 16 ops, theta 0.99, batch 65536", **out}
+                       "wd": shape["warp_density"], "bs": shape["block_size"], "window": shape["window"],
+                       "plain_tuned": [pshape["warp_density"], pshape["block_size"], pshape["batch_size"]]}
+            _progress(f"theta99 r{rr} {sc}: persistent {pv / 1e6:.2f} M, plain {qv / 1e6:.2f} M Tx/s ({pv / qv:.2f}x), "
+                      f"plain tuned {tv / 1e6:.2f} M ({pv / tv:.2f}x)")
+        out[f"read_{rr}"] = {"per_scheme": res, "min_ratio": min(v["ratio"] for v in res.values()) if res else None,
+                             "min_ratio_vs_tuned_plain": (min(v["ratio_vs_tuned_plain"] for v in res.values())
+                                                          if res else None)}
+    return {"workload": f"{args.rows} rows, {args.txs} tx x 16 ops, theta 0.99, batch 65536 (plain tuned: its own "
+                        f"batch size)", **out}
 
 
 SERVE_POOL_BYTES = 32 << 30  # the serving configuration's MVCC version-pool cap
