@@ -44,6 +44,7 @@ class Config(C.Structure):
         ("persist_adaptive", C.c_int32),
         ("row_ops", C.c_int32),
         ("debug_events", C.c_int32),
+        ("tpcc", C.c_int32),
     ]
 
 
@@ -86,6 +87,11 @@ class Workload(C.Structure):
         ("len_min", C.c_uint32), ("len_max", C.c_uint32), ("theta", C.c_double),
         ("read_ratio", C.c_double), ("seed", C.c_uint64), ("threads", C.c_int32),
     ]
+
+
+class TpccWorkload(C.Structure):
+    _fields_ = [("warehouses", C.c_uint32), ("new_order", C.c_int32), ("remote_ratio", C.c_double),
+                ("ntx", C.c_uint64), ("seed", C.c_uint64)]
 
 
 class StaticRun(C.Structure):
@@ -148,6 +154,14 @@ def load_library() -> C.CDLL:
                                     C.POINTER(Stats)]),
         "gcctb_check_host": (C.c_int, [C.POINTER(Config), _u64p, _u64p, _u64p, _u64p, C.c_uint64, _u64p, _u64p,
                                        C.POINTER(C.c_uint64)]),
+        "gcctb_load_params": (C.c_int, [P, _u32p, C.c_uint64]),
+        "gcctb_tpcc_rows": (C.c_uint64, [C.c_uint32]),
+        "gcctb_tpcc_seed_tables": (C.c_int, [C.c_uint32, C.c_uint64, _u64p, P]),
+        "gcctb_tpcc_generate": (C.c_int, [C.POINTER(TpccWorkload), _u64p, _u32p, _u64p, P, C.POINTER(C.c_uint64)]),
+        "gcctb_tpcc_new_order_totals": (C.c_int, [C.c_uint32, _u64p, P, _u64p, _u32p, _u64p, C.c_uint64, _u64p,
+                                                  _u64p, P, P]),
+        "gcctb_tpcc_check": (C.c_int, [C.c_uint32, _u64p, _u64p, _u64p, _u32p, _u64p, C.c_uint64, _u64p, _u64p,
+                                       C.POINTER(C.c_uint64)]),
         "gcctb_format_result": (C.c_int, [C.POINTER(Stats), C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                           C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint64]),
         "gcctb_device_info": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
@@ -207,6 +221,73 @@ def generate(rows: int, ntx: int, *, theta: float, read_ratio: float, len_min: i
     n = C.c_uint64(0)
     _check(lib.gcctb_generate(C.byref(w), ops, off, C.byref(n)))
     return Ops(ops[: n.value].copy() if n.value else np.zeros(0, dtype=np.uint64), off)
+
+
+class TpccOps(Ops):
+    """TPC-C transactions: CSR ops of the TPC-C row space, one update
+    parameter per op, and each transaction's customer row."""
+
+    def __init__(self, ops, tx_off, params, tx_customer):
+        super().__init__(ops, tx_off)
+        self.params = params
+        self.tx_customer = tx_customer
+
+
+TPCC_COLS = 5
+
+
+def tpcc_rows(warehouses: int) -> int:
+    return int(load_library().gcctb_tpcc_rows(warehouses))
+
+
+def tpcc_seed_tables(warehouses: int, seed: int = SEED) -> tuple:
+    """(initial cells of the TPC-C row space, item prices)."""
+    cells = np.zeros(tpcc_rows(warehouses) * TPCC_COLS, dtype=np.uint64)
+    price = np.zeros(100000, dtype=np.float32)
+    _check(load_library().gcctb_tpcc_seed_tables(warehouses, seed, cells, price.ctypes.data_as(C.c_void_p)))
+    return cells, price
+
+
+def tpcc_generate(warehouses: int, ntx: int, *, new_order: bool, remote_ratio: float = 0.1,
+                  seed: int = SEED) -> TpccOps:
+    """Deterministic payment or new_order transactions (generation/gen_tpcc.cu)."""
+    per = 16 if new_order else 3
+    ops = np.zeros(max(1, ntx * per), dtype=np.uint64)
+    params = np.zeros(max(1, ntx * per), dtype=np.uint32)
+    off = np.zeros(ntx + 1, dtype=np.uint64)
+    cust = np.zeros(max(1, ntx), dtype=np.uint64)
+    n = C.c_uint64(0)
+    wl = TpccWorkload(warehouses, 1 if new_order else 0, remote_ratio, ntx, seed)
+    _check(load_library().gcctb_tpcc_generate(C.byref(wl), ops, params, off, cust.ctypes.data_as(C.c_void_p),
+                                               C.byref(n)))
+    return TpccOps(ops[: n.value].copy(), off, params[: n.value].copy(), cust[:ntx].copy())
+
+
+def tpcc_new_order_totals(warehouses: int, init_cells: np.ndarray, price: np.ndarray, w: TpccOps,
+                          results: np.ndarray) -> tuple:
+    """new_order's output: (total amount per order, o_id per order)."""
+    total = np.zeros(max(1, w.ntx), dtype=np.float32)
+    oid = np.zeros(max(1, w.ntx), dtype=np.uint32)
+    _check(load_library().gcctb_tpcc_new_order_totals(
+        warehouses, np.ascontiguousarray(init_cells), np.ascontiguousarray(price, dtype=np.float32).ctypes.data_as(
+            C.c_void_p), np.ascontiguousarray(w.ops), np.ascontiguousarray(w.params), np.ascontiguousarray(w.tx_off),
+        w.ntx, np.ascontiguousarray(w.tx_customer), np.ascontiguousarray(results, dtype=np.uint64),
+        total.ctypes.data_as(C.c_void_p), oid.ctypes.data_as(C.c_void_p)))
+    return total[: w.ntx], oid[: w.ntx]
+
+
+def tpcc_check(warehouses: int, init_cells: np.ndarray, final_cells: np.ndarray, w: TpccOps, order: np.ndarray,
+               results: np.ndarray) -> int:
+    """Serial re-execution of a TPC-C run in its reported order: the count of
+    op results and cells that differ, plus tied order keys (0: the run is
+    the serial execution in that order)."""
+    bad = C.c_uint64(0)
+    a64 = lambda x: np.ascontiguousarray(x, dtype=np.uint64) if np.size(x) else np.zeros(1, np.uint64)
+    _check(load_library().gcctb_tpcc_check(
+        warehouses, a64(init_cells), a64(final_cells), a64(w.ops),
+        np.ascontiguousarray(w.params, dtype=np.uint32) if w.params.size else np.zeros(1, np.uint32), a64(w.tx_off),
+        w.ntx, a64(order), a64(results), C.byref(bad)))
+    return bad.value
 
 
 def check_ops(w: Ops, cols: int = 10) -> int:
@@ -349,13 +430,13 @@ class Context:
     def __init__(self, cc_scheme: str, *, rows: int, cols: int = 10, warp_density: int = 5,
                  block_size: int = 32, batch_size: int = 65536, seed: int = SEED,
                  backoff_ticks: int = 20, persist_window: int = 0, persist_adaptive: bool = False,
-                 row_ops: bool = False, debug_events: bool = False):
+                 row_ops: bool = False, debug_events: bool = False, tpcc: bool = False):
         self._lib = load_library()
         self.scheme = cc_scheme
         self.rows, self.cols, self.seed, self.row_ops = rows, cols, seed, row_ops
         cfg = Config(cc_scheme.encode(), warp_density, block_size, batch_size, rows, cols, seed,
                      backoff_ticks, persist_window, 1 if persist_adaptive else 0, 1 if row_ops else 0,
-                     1 if debug_events else 0)
+                     1 if debug_events else 0, 1 if tpcc else 0)
         h = C.c_void_p()
         _check(self._lib.gcctb_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -403,6 +484,11 @@ class Context:
             ops = np.zeros(1, dtype=np.uint64)
         _check(self._lib.gcctb_load_ops(self._h, ops, np.ascontiguousarray(w.tx_off, dtype=np.uint64), w.ntx))
         self.ntx, self.nops = w.ntx, w.nops
+
+    def load_params(self, params: np.ndarray) -> None:
+        """TPC-C: one update parameter per loaded op."""
+        p = np.ascontiguousarray(params, dtype=np.uint32)
+        _check(self._lib.gcctb_load_params(self._h, p if p.size else np.zeros(1, dtype=np.uint32), p.size))
 
     def run(self, form: str = "plain", check: bool = True) -> dict:
         """form: 'plain' (per-batch launches), 'graph' (captured batch sequence)

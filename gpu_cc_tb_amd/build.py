@@ -40,12 +40,13 @@ DEVICE_SRCS = [
     "device/exec_evt.hip",
     "device/exec_row_evt.hip",
     "device/exec_latch.hip",
+    "device/exec_tpcc.hip",
     "device/microbench.hip",
     "device/noalloc.hip",
 ]
 HOST_SRCS = ["host/engine.cpp", "host/workload.cpp", "host/check.cpp", "host/output.cpp",
-             "host/cpu_path.cpp"]
-TOOLS = {"test_ycsb": "tools/test_ycsb.cpp", "gen_ycsb": "tools/gen_ycsb.cpp"}
+             "host/cpu_path.cpp", "host/tpcc.cpp"]
+TOOLS = {"test_ycsb": "tools/test_ycsb.cpp", "gen_ycsb": "tools/gen_ycsb.cpp", "test_tpcc": "tools/test_tpcc.cpp"}
 ORACLE_SRCS = ["oracle_core.c", "cpu_cc.c"]
 
 

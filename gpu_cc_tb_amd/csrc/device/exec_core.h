@@ -39,9 +39,17 @@
 #ifndef GCCTB_LATCH
 #define GCCTB_LATCH 0
 #endif
+// TPC-C (GCCTB_TPCC, with GCCTB_ROW_OPS for whole-row versions): every op is a
+// read-modify-write of its row (tpcc_apply), namespace dev_tpcc.
+#ifndef GCCTB_TPCC
+#define GCCTB_TPCC 0
+#endif
 
 namespace gcctb {
-#if GCCTB_LATCH
+#if GCCTB_TPCC
+namespace dev_tpcc {
+using namespace dev;
+#elif GCCTB_LATCH
 namespace dev_latch {
 using namespace dev;
 #elif GCCTB_ROW_OPS && GCCTB_EVENTS
@@ -60,7 +68,9 @@ namespace dev {
 constexpr bool kRowOps = GCCTB_ROW_OPS != 0;
 constexpr bool kEvents = GCCTB_EVENTS != 0;
 constexpr bool kLatch = GCCTB_LATCH != 0;
+constexpr bool kTpcc = GCCTB_TPCC != 0;
 static_assert(!(kLatch && (kRowOps || kEvents)), "the latch schemes run in column mode without the event log");
+static_assert(!kTpcc || (kRowOps && !kEvents && !kLatch), "the TPC-C kernels move whole rows");
 
 // per-op slot flags
 constexpr uint32_t M_VALID = 1u;   // op exists and its key was found in the index
@@ -184,8 +194,22 @@ __device__ __forceinline__ uint64_t op_load(const ExecArgs& a, uint32_t row, uin
   else
     return row_digest(a.cells + (uint64_t)row * a.cols, a.cols);
 }
-__device__ __forceinline__ void op_store(const ExecArgs& a, uint32_t row, uint32_t meta, uint64_t tx, uint32_t j) {
-  if constexpr (!kRowOps) {
+// A write takes effect: the op's value (column mode), its whole row (row
+// mode), or, in the TPC-C kernels, its update applied to the row it reads
+// here, under the exclusive access the scheme holds; the update's result goes
+// to reads[op], op = the op's index in the op list.
+__device__ __forceinline__ void op_store(const ExecArgs& a, uint32_t row, uint32_t meta, uint64_t tx, uint32_t j,
+                                         uint64_t op) {
+  if constexpr (kTpcc) {
+    uint64_t* r = a.cells + (uint64_t)row * TPCC_COLS;
+    uint64_t v[TPCC_COLS];
+#pragma unroll
+    for (uint32_t i = 0; i < TPCC_COLS; ++i) v[i] = ld(r + i);
+    const uint64_t res = tpcc_apply(meta >> M_COL_SHIFT, a.op_param[op], v);
+#pragma unroll
+    for (uint32_t i = 0; i < TPCC_COLS; ++i) st(r + i, v[i]);
+    a.reads[op] = res;
+  } else if constexpr (!kRowOps) {
     st(cell_ptr(a, row, meta), write_value(a.seed, tx, j));
   } else {
     uint64_t* r = a.cells + (uint64_t)row * a.cols;
@@ -444,7 +468,7 @@ struct TwoPL {
         v[k] = 0;
         if (!(m & M_VALID)) continue;
         if (m & M_WRITE)
-          op_store(a, c.row[k], m, c.tx, op_index<G>(k));
+          op_store(a, c.row[k], m, c.tx, op_index<G>(k), c.op0 + op_index<G>(k));
         else
           v[k] = op_load(a, c.row[k], m);
       }
@@ -661,7 +685,7 @@ struct TimestampOrdering {
     // commit (to.cuh:117-134)
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k), c.op0 + op_index<G>(k));
     ev_commit<G>(c, a);
     order_release();
 #pragma unroll
@@ -894,7 +918,7 @@ struct Mvcc {
     if constexpr (kRowOps) order_release();  // the old row is saved before it is overwritten
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k), c.op0 + op_index<G>(k));
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -1088,7 +1112,7 @@ struct Silo {
     ev_commit<G>(c, a);
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k), c.op0 + op_index<G>(k));
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -1223,7 +1247,7 @@ struct Tictoc {
     ev_commit<G>(c, a);
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k));
+      if (c.meta[k] & M_HELD) op_store(a, c.row[k], c.meta[k], c.tx, op_index<G>(k), c.op0 + op_index<G>(k));
     order_release();
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -1287,7 +1311,7 @@ struct Ordered {
         const uint32_t m = c.meta[k];
         const uint32_t j = op_index<G>(k);
         if (m & M_WRITE)
-          op_store(a, c.row[k], m, c.tx, j);
+          op_store(a, c.row[k], m, c.tx, j, c.op0 + j);
         else
           a.reads[c.op0 + j] = op_load(a, c.row[k], m);
         ev_op<G>(c, a, k);

@@ -6,7 +6,9 @@
 #include "launch.h"
 
 namespace gcctb {
-#if GCCTB_LATCH
+#if GCCTB_TPCC
+namespace dev_tpcc {
+#elif GCCTB_LATCH
 namespace dev_latch {
 #elif GCCTB_ROW_OPS && GCCTB_EVENTS
 namespace dev_row_evt {
@@ -81,5 +83,5 @@ hipError_t launch_exec_g(int G, const ExecArgs& args, uint64_t nslots, int threa
 #undef GC_CASE
 }
 
-}  // namespace dev (dev_row, dev_evt, dev_row_evt, dev_latch)
+}  // namespace dev (dev_row, dev_evt, dev_row_evt, dev_latch, dev_tpcc)
 }  // namespace gcctb

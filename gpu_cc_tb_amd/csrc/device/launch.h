@@ -24,6 +24,8 @@ hipError_t exec_row_evt(int scheme, int G, const ExecArgs& a, uint64_t nslots, i
 // the latch schemes (slow_*), plain form, column mode: the CC arrays hold a
 // 16-byte slot per row (the word, then the latch)
 hipError_t exec_latch(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
+// TPC-C kernels (plain form): every op updates its row in place (tpcc_apply)
+hipError_t exec_tpcc(int scheme, int G, const ExecArgs& a, uint64_t nslots, int threads, hipStream_t s);
 
 // --- table / index ---
 hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t seed, hipStream_t s);

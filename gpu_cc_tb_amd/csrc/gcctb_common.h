@@ -305,6 +305,109 @@ struct ExecArgs {
   uint64_t* tcommit;       // optional, per tx: wall-clock tick of its commit (serving form)
   uint64_t* ev_op;         // event-log kernels: per op, the stamp of its access (0 = none)
   uint64_t* ev_commit;     // event-log kernels: per tx, the stamp of its commit (0 = none)
+  const uint32_t* op_param; // TPC-C kernels: per op, the update's parameter
 };
+
+// ---------------------------------------------------------------------------
+// TPC-C (benchmark/tpcc.cu, include/tpcc.cuh): payment and new_order over the
+// tables that carry concurrency control, in one row space of TPCC_COLS u64
+// cells per row:
+//   warehouse w                 row w                    (W rows)
+//   district (w, d)             W + w * 10 + d           (10 W)
+//   customer (w, d, c)          11 W + (w * 10 + d) * 3000 + c   (30,000 W)
+//   stock (w, i)                30,011 W + w * 100,000 + i       (100,000 W)
+// Item is read-only (a price per item, no concurrency control); the insert
+// tables (history, order, new_order, order_line) are derived from each
+// transaction's inputs and its ops' results.  Each 32-bit field sits in one
+// half of a cell (low half first):
+//   warehouse  c0 {id, tax}  c1 {ytd, -}
+//   district   c0 {id, w_id}  c1 {tax, ytd}  c2 {next_o_id, -}
+//   customer   c0 {id, d_id}  c1 {w_id, since}  c2 {credit_lim, discount}
+//              c3 {balance, ytd_payment}  c4 {payment_cnt, delivery_cnt}
+//   stock      c0 {i_id, w_id}  c1 {quantity, ytd}  c2 {order_cnt, remote_cnt}
+// An op reads its row, applies its update to it and writes it back while the
+// scheme gives it exclusive access (write lock, uncommitted mark, locked write
+// set, or its turn); the op's result is the field value the reference's
+// transaction reads back (next_o_id, the stock quantity) or the float it
+// updated first.
+// ---------------------------------------------------------------------------
+constexpr uint32_t TPCC_DISTRICTS = 10;
+constexpr uint32_t TPCC_CUSTOMERS = 3000;
+constexpr uint32_t TPCC_ITEMS = 100000;
+constexpr uint32_t TPCC_COLS = 5;
+constexpr uint32_t TPCC_MAX_LINES = 15;
+GC_HD uint64_t tpcc_rows(uint64_t w) { return w * (1 + TPCC_DISTRICTS + TPCC_DISTRICTS * TPCC_CUSTOMERS + TPCC_ITEMS); }
+GC_HD uint64_t tpcc_district_row(uint64_t W, uint64_t w, uint64_t d) { return W + w * TPCC_DISTRICTS + d; }
+GC_HD uint64_t tpcc_customer_row(uint64_t W, uint64_t w, uint64_t d, uint64_t c) {
+  return W * (1 + TPCC_DISTRICTS) + (w * TPCC_DISTRICTS + d) * TPCC_CUSTOMERS + c;
+}
+GC_HD uint64_t tpcc_stock_row(uint64_t W, uint64_t w, uint64_t i) {
+  return W * (1 + TPCC_DISTRICTS + TPCC_DISTRICTS * TPCC_CUSTOMERS) + w * TPCC_ITEMS + i;
+}
+// op kinds (the op's column field, below TPCC_COLS)
+enum TpccOp : uint32_t {
+  TPCC_PAY_W = 0,  // warehouse ytd += amount                        (tpcc.cu:89-103)
+  TPCC_PAY_D = 1,  // district ytd += amount                         (tpcc.cu:109-133)
+  TPCC_PAY_C = 2,  // customer balance -= amount, ytd_payment += amount, payment_cnt += 1 (tpcc.cu:139-163)
+  TPCC_NO_D = 3,   // district next_o_id += 1, result the old value  (tpcc.cu:259-283)
+  TPCC_NO_S = 4,   // stock quantity / ytd / order_cnt / remote_cnt  (tpcc.cu:298-337)
+};
+// param: the amount's float bits (payment), or quantity | remote << 16 (stock)
+GC_HD uint32_t tpcc_lo(uint64_t c) { return (uint32_t)c; }
+GC_HD uint32_t tpcc_hi(uint64_t c) { return (uint32_t)(c >> 32); }
+GC_HD uint64_t tpcc_pack(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
+GC_HD float tpcc_f(uint32_t bits) {
+  union {
+    uint32_t u;
+    float f;
+  } x{bits};
+  return x.f;
+}
+GC_HD uint32_t tpcc_bits(float f) {
+  union {
+    float f;
+    uint32_t u;
+  } x{f};
+  return x.u;
+}
+// Applies op `kind` with `param` to the row's cells r[0..TPCC_COLS) in place
+// and returns the op's result.  Float updates are single float additions in
+// the reference's order, so a serial replay reproduces them bit for bit.
+GC_HD uint64_t tpcc_apply(uint32_t kind, uint32_t param, uint64_t* r) {
+  switch (kind) {
+    case TPCC_PAY_W: {
+      const uint32_t old = tpcc_lo(r[1]);
+      r[1] = tpcc_pack(tpcc_bits(tpcc_f(old) + tpcc_f(param)), tpcc_hi(r[1]));
+      return old;
+    }
+    case TPCC_PAY_D: {
+      const uint32_t old = tpcc_hi(r[1]);
+      r[1] = tpcc_pack(tpcc_lo(r[1]), tpcc_bits(tpcc_f(old) + tpcc_f(param)));
+      return old;
+    }
+    case TPCC_PAY_C: {
+      const float amount = tpcc_f(param);
+      const uint32_t bal = tpcc_lo(r[3]), ytd = tpcc_hi(r[3]);
+      r[3] = tpcc_pack(tpcc_bits(tpcc_f(bal) - amount), tpcc_bits(tpcc_f(ytd) + amount));
+      r[4] = tpcc_pack(tpcc_lo(r[4]) + 1u, tpcc_hi(r[4]));
+      return bal;
+    }
+    case TPCC_NO_D: {
+      const uint32_t old = tpcc_lo(r[2]);
+      r[2] = tpcc_pack(old + 1u, tpcc_hi(r[2]));
+      return old;
+    }
+    case TPCC_NO_S: {
+      const uint32_t qty = param & 0xffffu, remote = param >> 16;
+      const uint32_t q = tpcc_lo(r[1]);
+      const uint32_t nq = q > qty + 10u ? q - qty : q - qty + 91u;
+      r[1] = tpcc_pack(nq, tpcc_hi(r[1]) + qty);
+      r[2] = tpcc_pack(tpcc_lo(r[2]) + 1u, tpcc_hi(r[2]) + remote);
+      return q;
+    }
+    default:
+      return 0;
+  }
+}
 
 }  // namespace gcctb

@@ -678,6 +678,7 @@ extern "C" int gcctb_cpu_run(const gcctb_config* cfg, int32_t threads, uint64_t*
                              gcctb_stats* st) {
   if (!cfg || !cfg->cc_scheme || !table || !tx_off || !reads || !order || !st || (!ops && ntx))
     return set_error(GCCTB_EINVAL, "null argument");
+  if (cfg->tpcc) return set_error(GCCTB_EINVAL, "the CPU path runs the YCSB transactions");
   const int scheme = scheme_from_name(cfg->cc_scheme);
   if (scheme < 0) return set_error(GCCTB_EINVAL, std::string("unknown cc_scheme ") + cfg->cc_scheme);
   if (cfg->rows == 0 || cfg->rows > 0x7fffffffull) return set_error(GCCTB_EINVAL, "rows out of range");

@@ -89,6 +89,9 @@ class SchemeHost;
 struct gcctb_ctx {
   gcctb::Scheme scheme;
   bool latch = false;  // a slow_* scheme: CC-word accesses under per-row latches
+  bool tpcc = false;   // TPC-C: every op updates its row (tpcc_apply)
+  gcctb::DevBuf<uint32_t> op_param;  // TPC-C: per-op update parameters
+  uint64_t params_loaded = 0;        // TPC-C: 1 + the op count the parameters were loaded for (0: none)
   gcctb_config cfg;
   int G = 1;
   int threads = 1024;
@@ -252,6 +255,7 @@ class SchemeHost {
     a.pool = ctx.pool.p;
     a.pool_cap = ctx.pool_cap;
     a.counters = ctx.counters.p;
+    a.op_param = ctx.op_param.p;
     a.slot_shift = slot_shift(ctx.cfg.rows);
     a.need = ctx.need.p;
     a.ctrs = ctx.ctrs.p;
@@ -282,11 +286,13 @@ class SchemeHost {
   // Kernel variants (whole-row mode, the event log): every scheme's plain
   // kernels of a variant come from one launcher.
   bool launch_rows(const ExecArgs& a, uint64_t n, bool persist) {
-    if (!ctx.cfg.row_ops && !ctx.cfg.debug_events && !ctx.latch) return false;
+    if (!ctx.cfg.row_ops && !ctx.cfg.debug_events && !ctx.latch && !ctx.tpcc) return false;
     if (persist)
       throw HipFailure{hipErrorInvalidValue, "whole-row mode, the event log and the latch schemes have no persistent kernels"};
     const int sch = (int)ctx.scheme;
-    if (ctx.latch)
+    if (ctx.tpcc)
+      HIP_TRY(launch::exec_tpcc(sch, ctx.G, a, n, ctx.threads, ctx.stream));
+    else if (ctx.latch)
       HIP_TRY(launch::exec_latch(sch, ctx.G, a, n, ctx.threads, ctx.stream));
     else if (ctx.cfg.row_ops && ctx.cfg.debug_events)
       HIP_TRY(launch::exec_row_evt(sch, ctx.G, a, n, ctx.threads, ctx.stream));
@@ -637,6 +643,8 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   if (sch < 0) return set_error(GCCTB_EINVAL, std::string("unknown cc_scheme ") + (cfg->cc_scheme ? cfg->cc_scheme : "(null)"));
   if (latch_base >= 0 && (cfg->row_ops || cfg->debug_events))
     return set_error(GCCTB_EINVAL, "the latch schemes run in column mode without the event log");
+  if (cfg->tpcc && (latch_base >= 0 || cfg->row_ops || cfg->debug_events || cfg->cols != TPCC_COLS))
+    return set_error(GCCTB_EINVAL, "TPC-C runs the eight schemes on rows of 5 cells, without row_ops or the event log");
   // benchmark_cpu/ycsb.cu:97-98: warp_cnt > 32 || valid_txn_bitoffset > 5 -> -1
   if (cfg->block_size < 1 || cfg->block_size > 32 || cfg->warp_density < 0 || cfg->warp_density > 5)
     return set_error(GCCTB_EINVAL, "warp_density must be 0..5 and block_size 1..32");
@@ -658,6 +666,7 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   if (!c) return set_error(GCCTB_ENOMEM, "out of host memory");
   c->scheme = (Scheme)sch;
   c->latch = latch_base >= 0;
+  c->tpcc = cfg->tpcc != 0;
   c->cfg = *cfg;
   c->cfg.cc_scheme = nullptr;
   c->G = 1 << cfg->warp_density;
@@ -876,6 +885,7 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
     c->ntx = ntx;
     c->nops = nops;
     c->nreads = nreads;
+    c->params_loaded = 0;  // TPC-C parameters belong to the ops loaded before
     c->inputs_gen++;
     c->custom_batches.clear();
     c->h_tx_off.assign(tx_off, tx_off + ntx + 1);
@@ -904,7 +914,7 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
     }
     if (c->scheme == SCHEME_MVCC) {
       // node: ts word, link, the superseded cell (or row, in whole-row mode)
-      const uint64_t stride = c->cfg.row_ops ? 2 + (uint64_t)c->cfg.cols : 3;
+      const uint64_t stride = (c->cfg.row_ops || c->tpcc) ? 2 + (uint64_t)c->cfg.cols : 3;
       c->pool_cap = cap;
       c->pool.alloc(stride * cap + stride);
       // nodes are published before they are linked, but a zeroed pool keeps
@@ -937,6 +947,20 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
   } catch (const HipFailure& f) {
     return hip_fail(f);
   }
+  return GCCTB_OK;
+}
+
+int gcctb_load_params(gcctb_ctx* c, const uint32_t* params, uint64_t n) {
+  if (!c || (!params && n)) return set_error(GCCTB_EINVAL, "null argument");
+  if (!c->tpcc) return set_error(GCCTB_EINVAL, "op parameters belong to a TPC-C context");
+  if (n != c->nops) return set_error(GCCTB_EINVAL, "one parameter per loaded op");
+  try {
+    c->op_param.alloc(n ? n : 1);
+    if (n) HIP_TRY(hipMemcpy(c->op_param.p, params, n * 4, hipMemcpyHostToDevice));
+  } catch (const HipFailure& f) {
+    return hip_fail(f);
+  }
+  c->params_loaded = n + 1;
   return GCCTB_OK;
 }
 
@@ -984,8 +1008,11 @@ static std::string error_text(uint64_t flags) {
 static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
   if (!c) return set_error(GCCTB_EINVAL, "null context");
   if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
-  if (form == FORM_PERSIST && (c->cfg.row_ops || c->cfg.debug_events || c->latch))
-    return set_error(GCCTB_EINVAL, "whole-row mode, the event log and the latch schemes run the plain and graph forms");
+  if (form == FORM_PERSIST && (c->cfg.row_ops || c->cfg.debug_events || c->latch || c->tpcc))
+    return set_error(GCCTB_EINVAL, "whole-row mode, the event log, the latch schemes and TPC-C run the plain and graph forms");
+  // a TPC-C op reads its update's parameter: none may be missing
+  if (c->tpcc && c->params_loaded != c->nops + 1)
+    return set_error(GCCTB_ESTATE, "load the TPC-C op parameters (gcctb_load_params) after the ops");
   gcctb_stats st{};
   Metrics m{};
   try {
@@ -1210,8 +1237,8 @@ int gcctb_serve(gcctb_ctx* c, const gcctb_serve_config* sc, const uint64_t* batc
   if (!c->table_ready) return set_error(GCCTB_ESTATE, "load or seed the table first");
   if (c->scheme == SCHEME_GPUTX)
     return set_error(GCCTB_EINVAL, "gputx ranks its whole input before running; it has no serving form");
-  if (c->cfg.row_ops || c->cfg.debug_events || c->latch)
-    return set_error(GCCTB_EINVAL, "whole-row mode, the event log and the latch schemes run the plain and graph forms");
+  if (c->cfg.row_ops || c->cfg.debug_events || c->latch || c->tpcc)
+    return set_error(GCCTB_EINVAL, "whole-row mode, the event log, the latch schemes and TPC-C run the plain and graph forms");
   uint64_t total = 0;
   for (uint64_t i = 0; i < nbatches; ++i) {
     if (batch_sizes[i] == 0) return set_error(GCCTB_EINVAL, "empty batch");
@@ -1562,6 +1589,7 @@ int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int
 
 int gcctb_check(gcctb_ctx* c, const uint64_t* init_table, uint64_t* bad, uint64_t* uncommitted) {
   if (!c || !init_table || !bad) return set_error(GCCTB_EINVAL, "null argument");
+  if (c->tpcc) return set_error(GCCTB_EINVAL, "the check covers the YCSB ops; TPC-C runs are checked by replay");
   const uint64_t ncells = c->cfg.rows * c->cfg.cols;
   std::vector<uint64_t> fin(ncells), reads(c->nops + 1), order(3 * c->ntx + 1), ops(c->nops + 1);
   std::vector<uint32_t> rows(c->nops + 1);

@@ -54,6 +54,9 @@ typedef struct gcctb_config {
                                the row's digest; plain and graph forms only */
   int32_t debug_events;     /* 1: the plain and graph forms record the event log of
                                gcctb_fetch_events (the debug mode's AddEvent) */
+  int32_t tpcc;             /* 1: TPC-C (benchmark/tpcc.cu): rows of the TPC-C row space
+                               (cols 5, gcctb_tpcc_* below), every op an update of its row
+                               with a parameter from gcctb_load_params; plain and graph forms */
 } gcctb_config;
 
 /* Result of one run.  The first block mirrors the reference's Metrics and
@@ -115,6 +118,9 @@ int gcctb_probe(gcctb_ctx* ctx, const uint32_t* keys, uint64_t n, uint32_t* rows
  * are distinct (the reference's generator drops repeated keys,
  * generation/gen_ycsb.cu:129-136); a repeated key returns GCCTB_EINVAL. */
 int gcctb_load_ops(gcctb_ctx* ctx, const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx);
+/* TPC-C contexts: one update parameter per loaded op (gcctb_tpcc_generate
+ * writes them), loaded after gcctb_load_ops. */
+int gcctb_load_params(gcctb_ctx* ctx, const uint32_t* params, uint64_t n);
 /* The input checks of gcctb_load_ops on the host alone (no context, no GPU):
  * offsets, lengths, columns below cols, distinct keys per transaction, and
  * the op and transaction limits.  *nreads (may be null) gets the read count. */
@@ -337,6 +343,51 @@ int gcctb_cpu_run(const gcctb_config* cfg, int32_t threads, uint64_t* table, con
 int gcctb_check_host(const gcctb_config* cfg, const uint64_t* init_table, const uint64_t* final_table,
                      const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, const uint64_t* order,
                      const uint64_t* reads, uint64_t* bad);
+
+/* TPC-C (benchmark/tpcc.cu, benchmark_cpu/tpcc.cu, generation/gen_tpcc*.cu):
+ * payment and new_order over the tables that carry concurrency control,
+ * laid out as one row space of 5 u64 cells per row (warehouses, then
+ * districts, customers and stock; the layout and the field packing are
+ * stated in gcctb_common.h).  A context with gcctb_config.tpcc = 1, rows =
+ * gcctb_tpcc_rows(W) and cols = 5 runs them: gcctb_load_table with the rows
+ * of gcctb_tpcc_seed_tables, gcctb_load_ops + gcctb_load_params with
+ * gcctb_tpcc_generate's ops and parameters, then gcctb_run_plain (or
+ * gcctb_run_graph).  Every op updates its row under the scheme's exclusive
+ * access and leaves its result (the next_o_id or stock quantity it read, or
+ * the float it updated first) in the reads of gcctb_fetch. */
+typedef struct gcctb_tpcc_workload {
+  uint32_t warehouses;   /* W, 1..8192 */
+  int32_t new_order;     /* 1: new_order transactions, 0: payment (benchmark_cpu/tpcc.cu's isno) */
+  double remote_ratio;   /* new_order: chance that an order line's stock comes from another
+                            warehouse (gen_tpcc.cu's no_remote_ratio) */
+  uint64_t ntx;
+  uint64_t seed;
+} gcctb_tpcc_workload;
+uint64_t gcctb_tpcc_rows(uint32_t warehouses);
+/* The initial rows (gen_tpcc_table.cu's field values, seeded draws where it
+ * draws): cells holds gcctb_tpcc_rows(W) * 5 u64; item_price (may be null)
+ * 100,000 floats. */
+int gcctb_tpcc_seed_tables(uint32_t warehouses, uint64_t seed, uint64_t* cells, float* item_price);
+/* ntx transactions as CSR ops with one parameter per op: ops and params hold
+ * 3 (payment) or 16 (new_order) entries per transaction; tx_customer (may be
+ * null) gets each transaction's customer row. */
+int gcctb_tpcc_generate(const gcctb_tpcc_workload* w, uint64_t* ops, uint32_t* params, uint64_t* tx_off,
+                        uint64_t* tx_customer, uint64_t* nops);
+/* new_order's output (tpcc.cu:361-376): total[t] = sum of quantity x price
+ * over the order's lines x (1 - discount) x (1 + w_tax + d_tax), o_id[t]
+ * (may be null) = the next_o_id the order took, from a run's results. */
+int gcctb_tpcc_new_order_totals(uint32_t warehouses, const uint64_t* init_cells, const float* item_price,
+                                const uint64_t* ops, const uint32_t* params, const uint64_t* tx_off, uint64_t ntx,
+                                const uint64_t* tx_customer, const uint64_t* results, float* total, uint32_t* o_id);
+/* The debug check of a TPC-C run (the serial-replay half of ExecInfo::Check,
+ * src/runtime.cu:129-158, which the reference's TPC-C driver reaches with
+ * debug != 0): the transactions re-executed one by one in `order` (3 u64 per
+ * transaction, as gcctb_fetch returns it) from init_cells; *bad = op results
+ * and cells that differ from results and final_cells, plus transactions
+ * whose order keys tie. */
+int gcctb_tpcc_check(uint32_t warehouses, const uint64_t* init_cells, const uint64_t* final_cells,
+                     const uint64_t* ops, const uint32_t* params, const uint64_t* tx_off, uint64_t ntx,
+                     const uint64_t* order, const uint64_t* results, uint64_t* bad);
 
 /* Device information. */
 int gcctb_device_info(char* name, uint64_t name_len, double* wall_clock_khz, int32_t* cus);

@@ -83,6 +83,17 @@ int orc_cpu_run(int scheme, uint64_t rows, uint32_t cols, uint64_t seed, uint64_
                 const uint64_t* ops, const uint64_t* tx_off, uint64_t ntx, uint64_t batch_size, int threads,
                 uint64_t* order, uint64_t* reads, orc_cpu_stats* stats);
 
+/* ---- TPC-C (benchmark/tpcc.cu:36-380) ----
+ * One op's update of its 5-cell row (kind 0 warehouse, 1 district, 2
+ * customer for payment; 3 district, 4 stock for new_order; param the
+ * amount's float bits or quantity | remote << 16); returns the op's result
+ * (the field read back, or the float updated first). */
+uint64_t orc_tpcc_apply(uint32_t kind, uint32_t param, uint64_t* row);
+/* Serial replay of TPC-C transactions in (batch, key0, key1) order; returns
+ * the number of tied keys (-1: allocation failed). */
+int64_t orc_tpcc_replay(uint64_t rows, uint64_t* table, const uint64_t* ops, const uint32_t* params,
+                        const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -365,3 +365,98 @@ int orc_gputx_rank(uint64_t rows, const uint64_t* ops, const uint64_t* tx_off, u
   }
   return 0;
 }
+
+/* ---- TPC-C (benchmark/tpcc.cu) ----
+ * The updates payment and new_order make to the rows that carry concurrency
+ * control, on rows of 5 u64 cells, two 32-bit fields per cell (low half
+ * first): warehouse {id, tax} {ytd, -}; district {id, w_id} {tax, ytd}
+ * {next_o_id, -}; customer {id, d_id} {w_id, since} {credit_lim, discount}
+ * {balance, ytd_payment} {payment_cnt, delivery_cnt}; stock {i_id, w_id}
+ * {quantity, ytd} {order_cnt, remote_cnt}.  kind 0..4 and the result as in
+ * orc_tpcc_apply's comments. */
+static float f32_of(uint32_t u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+static uint32_t u32_of(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+#define LO32(c) ((uint32_t)(c))
+#define HI32(c) ((uint32_t)((c) >> 32))
+#define PACK32(lo, hi) ((uint64_t)(uint32_t)(lo) | ((uint64_t)(uint32_t)(hi) << 32))
+
+uint64_t orc_tpcc_apply(uint32_t kind, uint32_t param, uint64_t* r) {
+  switch (kind) {
+    case 0: { /* payment, warehouse.ytd += h_amount (tpcc.cu:100) */
+      const uint32_t old = LO32(r[1]);
+      const float ytd = f32_of(old) + f32_of(param);
+      r[1] = PACK32(u32_of(ytd), HI32(r[1]));
+      return old;
+    }
+    case 1: { /* payment, district.ytd += h_amount (tpcc.cu:125) */
+      const uint32_t old = HI32(r[1]);
+      const float ytd = f32_of(old) + f32_of(param);
+      r[1] = PACK32(LO32(r[1]), u32_of(ytd));
+      return old;
+    }
+    case 2: { /* payment, customer balance / ytd_payment / payment_cnt (tpcc.cu:152-154) */
+      const uint32_t bal = LO32(r[3]);
+      const float nb = f32_of(bal) - f32_of(param);
+      const float ny = f32_of(HI32(r[3])) + f32_of(param);
+      r[3] = PACK32(u32_of(nb), u32_of(ny));
+      r[4] = PACK32(LO32(r[4]) + 1u, HI32(r[4]));
+      return bal;
+    }
+    case 3: { /* new_order, cur_o_id = next_o_id; next_o_id++ (tpcc.cu:270-271) */
+      const uint32_t old = LO32(r[2]);
+      r[2] = PACK32(old + 1u, HI32(r[2]));
+      return old;
+    }
+    case 4: { /* new_order, stock update (tpcc.cu:314-322) */
+      const uint32_t qty = param & 0xffffu, remote = param >> 16;
+      const uint32_t q = LO32(r[1]);
+      uint32_t nq;
+      if (q > qty + 10u)
+        nq = q - qty;
+      else
+        nq = q - (qty - 91u);
+      r[1] = PACK32(nq, HI32(r[1]) + qty);
+      r[2] = PACK32(LO32(r[2]) + 1u, HI32(r[2]) + remote);
+      return q;
+    }
+    default:
+      return 0;
+  }
+}
+
+int64_t orc_tpcc_replay(uint64_t rows, uint64_t* table, const uint64_t* ops, const uint32_t* params,
+                        const uint64_t* tx_off, uint64_t ntx, const uint64_t* order, uint64_t* results) {
+  okey_t* ord = (okey_t*)malloc(sizeof(okey_t) * (ntx ? ntx : 1));
+  if (!ord) return -1;
+  for (uint64_t t = 0; t < ntx; ++t) {
+    ord[t].b = order[3 * t];
+    ord[t].k0 = order[3 * t + 1];
+    ord[t].k1 = order[3 * t + 2];
+    ord[t].tx = t;
+  }
+  qsort(ord, ntx, sizeof(okey_t), cmp_okey);
+  int64_t ties = 0;
+  for (uint64_t i = 1; i < ntx; ++i)
+    if (ord[i].b == ord[i - 1].b && ord[i].k0 == ord[i - 1].k0 && ord[i].k1 == ord[i - 1].k1) ties++;
+  for (uint64_t k = 0; k < ntx; ++k) {
+    const uint64_t t = ord[k].tx;
+    for (uint64_t i = tx_off[t]; i < tx_off[t + 1]; ++i) {
+      const uint64_t row = (uint32_t)ops[i];
+      if (row >= rows) {
+        results[i] = 0;
+        continue;
+      }
+      results[i] = orc_tpcc_apply((uint32_t)(ops[i] >> 32) & 0xff, params[i], table + row * 5);
+    }
+  }
+  free(ord);
+  return ties;
+}

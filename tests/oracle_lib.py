@@ -66,6 +66,10 @@ def lib() -> C.CDLL:
         L.orc_cpu_run.restype = C.c_int
         L.orc_cpu_run.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, _u64p, _u64p, _u64p, C.c_uint64,
                                   C.c_uint64, C.c_int, _u64p, _u64p, C.POINTER(CpuStats)]
+        L.orc_tpcc_apply.restype = C.c_uint64
+        L.orc_tpcc_apply.argtypes = [C.c_uint32, C.c_uint32, _u64p]
+        L.orc_tpcc_replay.restype = C.c_int64
+        L.orc_tpcc_replay.argtypes = [C.c_uint64, _u64p, _u64p, _u32p, _u64p, C.c_uint64, _u64p, _u64p]
         _lib = L
     return _lib
 
@@ -151,3 +155,21 @@ def cpu_run(scheme: int, rows, cols, seed, table, ops, tx_off, batch_size, threa
     rc = lib().orc_cpu_run(scheme, rows, cols, seed, t, _a64(ops), _a64(tx_off), ntx, batch_size, threads,
                            order, reads, C.byref(st))
     return rc, t, reads[: len(ops)], order[: 3 * ntx], st
+
+
+def tpcc_apply(kind: int, param: int, row) -> tuple:
+    """One TPC-C op on a 5-cell row: (result, updated row)."""
+    r = np.array(row, dtype=np.uint64).copy()
+    res = lib().orc_tpcc_apply(kind, param, r)
+    return int(res), r
+
+
+def tpcc_replay(rows, table, ops, params, tx_off, order):
+    """Serial TPC-C replay in (batch, key0, key1) order -> (final table, results, ties)."""
+    t = np.array(table, dtype=np.uint64, copy=True)
+    res = np.zeros(max(1, len(ops)), dtype=np.uint64)
+    p = np.ascontiguousarray(params, dtype=np.uint32)
+    ties = lib().orc_tpcc_replay(rows, t, _a64(ops), p if p.size else np.zeros(1, dtype=np.uint32), _a64(tx_off),
+                                 len(tx_off) - 1, _a64(order), res)
+    assert ties >= 0
+    return t, res[: len(ops)], ties
