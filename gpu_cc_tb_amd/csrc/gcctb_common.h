@@ -346,11 +346,11 @@ GC_HD uint64_t tpcc_stock_row(uint64_t W, uint64_t w, uint64_t i) {
 }
 // op kinds (the op's column field, below TPCC_COLS)
 enum TpccOp : uint32_t {
-  TPCC_PAY_W = 0,  // warehouse ytd += amount                        (tpcc.cu:89-103)
-  TPCC_PAY_D = 1,  // district ytd += amount                         (tpcc.cu:109-133)
-  TPCC_PAY_C = 2,  // customer balance -= amount, ytd_payment += amount, payment_cnt += 1 (tpcc.cu:139-163)
-  TPCC_NO_D = 3,   // district next_o_id += 1, result the old value  (tpcc.cu:259-283)
-  TPCC_NO_S = 4,   // stock quantity / ytd / order_cnt / remote_cnt  (tpcc.cu:298-337)
+  TPCC_PAY_W = 0,  // warehouse ytd += amount                        (tpcc.cu:89-110)
+  TPCC_PAY_D = 1,  // district ytd += amount                         (tpcc.cu:112-134)
+  TPCC_PAY_C = 2,  // customer balance -= amount, ytd_payment += amount, payment_cnt += 1 (tpcc.cu:136-163)
+  TPCC_NO_D = 3,   // district next_o_id += 1, result the old value  (tpcc.cu:255-283)
+  TPCC_NO_S = 4,   // stock quantity / ytd / order_cnt / remote_cnt  (tpcc.cu:301-338)
 };
 // param: the amount's float bits (payment), or quantity | remote << 16 (stock)
 GC_HD uint32_t tpcc_lo(uint64_t c) { return (uint32_t)c; }

@@ -11,7 +11,7 @@
 // reference's name (benchmark_cpu/tpcc.cu:453-455).  With debug != 0 the run
 // is re-executed serially on the host in the order the kernels reported
 // (gcctb_tpcc_check) and "REPLAY OK" or "REPLAY BAD <n>" follows; new_order
-// also prints the first orders' totals (tpcc.cu:361-376).
+// also prints the first orders' totals (tpcc.cu:340,371).
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>

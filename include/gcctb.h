@@ -373,7 +373,7 @@ int gcctb_tpcc_seed_tables(uint32_t warehouses, uint64_t seed, uint64_t* cells, 
  * null) gets each transaction's customer row. */
 int gcctb_tpcc_generate(const gcctb_tpcc_workload* w, uint64_t* ops, uint32_t* params, uint64_t* tx_off,
                         uint64_t* tx_customer, uint64_t* nops);
-/* new_order's output (tpcc.cu:361-376): total[t] = sum of quantity x price
+/* new_order's output (tpcc.cu:340,371-372): total[t] = sum of quantity x price
  * over the order's lines x (1 - discount) x (1 + w_tax + d_tax), o_id[t]
  * (may be null) = the next_o_id the order took, from a run's results. */
 int gcctb_tpcc_new_order_totals(uint32_t warehouses, const uint64_t* init_cells, const float* item_price,

@@ -390,7 +390,7 @@ static uint32_t u32_of(float f) {
 
 uint64_t orc_tpcc_apply(uint32_t kind, uint32_t param, uint64_t* r) {
   switch (kind) {
-    case 0: { /* payment, warehouse.ytd += h_amount (tpcc.cu:100) */
+    case 0: { /* payment, warehouse.ytd += h_amount (tpcc.cu:102) */
       const uint32_t old = LO32(r[1]);
       const float ytd = f32_of(old) + f32_of(param);
       r[1] = PACK32(u32_of(ytd), HI32(r[1]));
@@ -415,7 +415,7 @@ uint64_t orc_tpcc_apply(uint32_t kind, uint32_t param, uint64_t* r) {
       r[2] = PACK32(old + 1u, HI32(r[2]));
       return old;
     }
-    case 4: { /* new_order, stock update (tpcc.cu:314-322) */
+    case 4: { /* new_order, stock update (tpcc.cu:319-327) */
       const uint32_t qty = param & 0xffffu, remote = param >> 16;
       const uint32_t q = LO32(r[1]);
       uint32_t nq;

@@ -24,7 +24,7 @@ def pack(lo, hi) -> int:
 
 def test_oracle_updates_hand_worked():
     """Each update of benchmark/tpcc.cu on a row worked by hand."""
-    # payment, warehouse: ytd 300000 + 12.5 (tpcc.cu:100)
+    # payment, warehouse: ytd 300000 + 12.5 (tpcc.cu:102)
     res, r = orc.tpcc_apply(0, bits(12.5), [pack(3, bits(0.125)), pack(bits(300000.0), 0), 0, 0, 0])
     assert res == bits(300000.0) and r[1] == pack(bits(300012.5), 0) and r[0] == pack(3, bits(0.125))
     # payment, district: ytd (high half of cell 1) 30000 + 7.25, tax untouched (tpcc.cu:125)
@@ -36,7 +36,7 @@ def test_oracle_updates_hand_worked():
     # new_order, district: the order takes next_o_id 3001, the district moves to 3002 (tpcc.cu:270-271)
     res, r = orc.tpcc_apply(3, 0, [0, 0, pack(3001, 7), 0, 0])
     assert res == 3001 and r[2] == pack(3002, 7)
-    # new_order, stock (tpcc.cu:314-322): 50 > 5 + 10 -> 45; ytd + 5, order_cnt + 1, remote_cnt + 1
+    # new_order, stock (tpcc.cu:319-327): 50 > 5 + 10 -> 45; ytd + 5, order_cnt + 1, remote_cnt + 1
     res, r = orc.tpcc_apply(4, 5 | (1 << 16), [0, pack(50, 20), pack(3, 4), 0, 0])
     assert res == 50 and r[1] == pack(45, 25) and r[2] == pack(4, 5)
     # 15 is not > 5 + 10: 15 - (5 - 91) = 101 in unsigned arithmetic; local line
@@ -149,7 +149,7 @@ def test_new_order_generator():
 
 
 def test_new_order_totals_independent():
-    """new_order's totals (tpcc.cu:361-376) against float32 arithmetic done
+    """new_order's totals (tpcc.cu:340,371) against float32 arithmetic done
     here from the tables, the ops and a set of results."""
     W, n = 2, 50
     cells, price = g.tpcc_seed_tables(W)
