@@ -58,6 +58,7 @@ class Stats(C.Structure):
         ("batches", C.c_uint64), ("ranks", C.c_uint64), ("same_rank_conflicts", C.c_uint64),
         ("index_miss", C.c_uint64), ("error_flags", C.c_uint64), ("total_ops", C.c_uint64),
         ("read_ops", C.c_uint64),
+        ("deferred", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -116,6 +116,21 @@ hipError_t rank_hist(const uint32_t* rank, uint64_t ntx, uint32_t nranks, uint32
 hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint64_t iters, double* rate,
                       double* ms_out);
 
+// --- hot-writer deferral (persistent form of the non-ordered schemes) ---
+// map[slot] = the transaction the slot runs: the transactions that write no
+// hot key from slot 0 up, those that do from slot ntx - 1 down (a key is hot
+// when at least one transaction in `share` touches it, estimated from about
+// DEFER_SAMPLE sampled ops).  cnt holds DEFER_WORDS u32: DEFER_BUCKETS
+// sampled counts, then the state words (cnt + DEFER_BUCKETS)[DEFER_ANY] (a
+// key was hot) and [DEFER_NHOT] (the number deferred); map holds ntx u32.
+constexpr uint32_t DEFER_BUCKET_BITS = 16;
+constexpr uint32_t DEFER_BUCKETS = 1u << DEFER_BUCKET_BITS;
+constexpr uint64_t DEFER_SAMPLE = 16384;
+constexpr uint32_t DEFER_ANY = 0, DEFER_NCOLD = 1, DEFER_NHOT = 2;
+constexpr uint64_t DEFER_WORDS = DEFER_BUCKETS + 8;
+hipError_t defer_order(const uint64_t* tx_off, const uint64_t* ops, uint64_t ntx, uint64_t nops, uint32_t share,
+                       uint32_t* cnt, uint32_t* map, hipStream_t s);
+
 // --- per-batch state ---
 // clear the CC words (and MVCC chain heads) of rows[0, n) at their slots
 // (arrays of slot_words(nrows_table) words); latch: the CC array holds two

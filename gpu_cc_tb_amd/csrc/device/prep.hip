@@ -677,6 +677,100 @@ __global__ void k_clock_sample(uint64_t* out) {
   if (threadIdx.x == 0) dev::st(out, wall_clock64());
 }
 
+// ---------------------------------------------------------------------------
+// Hot-writer deferral (persistent form of the lock, timestamp and OCC
+// schemes).  A transaction that writes a hot key conflicts with every
+// concurrent transaction touching that key, and under high skew those writers
+// hold up (2PL) or abort (OCC, T/O) the far larger number of transactions that
+// only read the hot keys.  The launch therefore takes every other transaction
+// first and the hot writers last.  A key is hot when its bucket (Fibonacci
+// hash of the key) holds at least `thr` of a sample of the ops (every
+// stride-th op); thr is scaled so that a key touched by at least one
+// transaction in `share` qualifies.  The deferral only reorders admission:
+// each scheme still reports its own serialization order.
+__device__ __forceinline__ uint32_t defer_bucket(uint32_t key) {
+  return (key * 2654435761u) >> (32 - launch::DEFER_BUCKET_BITS);
+}
+
+// sampled access counts per bucket; the lanes of a wave that share a bucket
+// add together (one atomic per distinct bucket of the wave)
+__global__ __launch_bounds__(BT) void k_defer_count(const uint64_t* ops, uint64_t nsamp, uint64_t stride,
+                                                    uint32_t* cnt) {
+  for (uint64_t base = (uint64_t)blockIdx.x * BT; base < nsamp; base += (uint64_t)gridDim.x * BT) {
+    const uint64_t i = base + threadIdx.x;
+    const bool v = i < nsamp;
+    const uint32_t b = v ? defer_bucket(op_key(ops[i * stride])) : 0u;
+    uint64_t act = dev::ballot(v);
+    while (act) {
+      const int l = __ffsll((unsigned long long)act) - 1;
+      const uint32_t lb = dev::shfl32(b, l);
+      const uint64_t same = dev::ballot(b == lb) & act;
+      if ((int)dev::lane_id() == l) atomicAdd(cnt + lb, (uint32_t)__popcll(same));
+      act &= ~same;
+    }
+  }
+}
+
+// any bucket hot?  (st was cleared with the counts; every hit writes 1)
+__global__ __launch_bounds__(BT) void k_defer_any(const uint32_t* cnt, uint32_t thr, uint32_t* st) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i < launch::DEFER_BUCKETS && cnt[i] >= thr) st[launch::DEFER_ANY] = 1u;
+}
+
+// map[slot] = transaction.  Nothing hot: the identity.  Otherwise a
+// transaction that writes a hot key is deferred: the others fill the slots
+// from 0 up, the deferred ones from ntx - 1 down, each class in the order its
+// blocks claim their ranges (one pair of atomics per block step);
+// st[DEFER_NHOT] ends as the number deferred.
+__global__ __launch_bounds__(BT) void k_defer_order(const uint64_t* tx_off, const uint64_t* ops, uint64_t ntx,
+                                                    const uint32_t* cnt, uint32_t thr, uint32_t* st, uint32_t* map) {
+  __shared__ uint32_t wc[BT / 64], wh[BT / 64];
+  __shared__ uint32_t bc, bh;
+  const bool any = st[launch::DEFER_ANY] != 0;  // the same for every thread of the grid
+  const uint32_t w = threadIdx.x / 64;
+  for (uint64_t base = (uint64_t)blockIdx.x * BT; base < ntx; base += (uint64_t)gridDim.x * BT) {
+    const uint64_t t = base + threadIdx.x;
+    const bool v = t < ntx;
+    if (!any) {
+      if (v) map[t] = (uint32_t)t;
+      continue;
+    }
+    bool h = false;
+    if (v) {
+      const uint64_t o1 = tx_off[t + 1];
+      for (uint64_t o = tx_off[t]; o < o1; ++o) {
+        const uint64_t op = ops[o];
+        h |= op_is_write(op) && cnt[defer_bucket(op_key(op))] >= thr;
+      }
+    }
+    const uint64_t mh = dev::ballot(h), mc = dev::ballot(v && !h);
+    if (dev::lane_id() == 0) {
+      wc[w] = (uint32_t)__popcll(mc);
+      wh[w] = (uint32_t)__popcll(mh);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t sc = 0, sh = 0;
+      for (int i = 0; i < BT / 64; ++i) {
+        const uint32_t x = wc[i], y = wh[i];
+        wc[i] = sc;
+        wh[i] = sh;
+        sc += x;
+        sh += y;
+      }
+      bc = sc ? atomicAdd(st + launch::DEFER_NCOLD, sc) : 0u;
+      bh = sh ? atomicAdd(st + launch::DEFER_NHOT, sh) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = dev::lanemask_lt();
+    if (h)
+      map[ntx - 1 - (bh + wh[w] + (uint32_t)__popcll(mh & below))] = (uint32_t)t;
+    else if (v)
+      map[bc + wc[w] + (uint32_t)__popcll(mc & below)] = (uint32_t)t;
+    __syncthreads();  // wc / wh / bc / bh are rewritten by the next step
+  }
+}
+
 }  // namespace
 
 namespace launch {
@@ -713,6 +807,27 @@ hipError_t op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out,
   GC_LAUNCH(k_op_ids, n, ops, n, base, out);
 }
 hipError_t fill_u32(uint32_t* v, uint64_t n, uint32_t x, hipStream_t s) { GC_LAUNCH(k_fill32, n, v, n, x); }
+hipError_t defer_order(const uint64_t* tx_off, const uint64_t* ops, uint64_t ntx, uint64_t nops, uint32_t share,
+                       uint32_t* cnt, uint32_t* map, hipStream_t s) {
+  if (ntx == 0) return hipSuccess;
+  // about DEFER_SAMPLE sampled ops
+  const uint64_t stride = nops > DEFER_SAMPLE ? nops / DEFER_SAMPLE : 1;
+  const uint64_t nsamp = nops / stride;
+  const uint64_t want = (ntx + (uint64_t)share * stride - 1) / ((uint64_t)share * stride);
+  const uint32_t thr = (uint32_t)(want < 2 ? 2 : want > 0xffffffffull ? 0xffffffffull : want);
+  uint32_t* st = cnt + DEFER_BUCKETS;
+  hipLaunchKernelGGL(k_fill32, dim3(grid_stride_blocks(DEFER_WORDS)), dim3(BT), 0, s, cnt, (uint64_t)DEFER_WORDS, 0u);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (nsamp) {
+    hipLaunchKernelGGL(k_defer_count, dim3(grid_stride_blocks(nsamp)), dim3(BT), 0, s, ops, nsamp, stride, cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_defer_any, dim3(DEFER_BUCKETS / BT), dim3(BT), 0, s, cnt, thr, st);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_defer_order, dim3(reduce_blocks(ntx)), dim3(BT), 0, s, tx_off, ops, ntx, cnt, thr, st, map);
+  return hipGetLastError();
+}
 hipError_t fill_u64(uint64_t* v, uint64_t n, uint64_t x, hipStream_t s) { GC_LAUNCH(k_fill64, n, v, n, x); }
 
 uint64_t scan_tmp_words(uint64_t n) { return (n + SC_TILE - 1) / SC_TILE + 64; }

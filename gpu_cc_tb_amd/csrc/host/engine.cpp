@@ -103,6 +103,12 @@ struct gcctb_ctx {
   int probe_pass = -1;
   uint32_t backoff_exp = 10;
   uint32_t poll_ticks = 10;  // 0.1 us (measured at theta 0.99 with the hot rows' turn counters spread)
+  // persistent form of the lock, timestamp and OCC schemes: transactions that
+  // write a key touched by at least one transaction in defer_share run after
+  // the others.  GCCTB_DEFER_SHARE sets it for every scheme (0 turns it
+  // off); by default (-1) 16 for 2PL, T/O and MVCC, and off for Silo and
+  // TicToc, which measured no gain from it
+  int defer_share = -1;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   // pinned host word: the error flags of the launches since the last metrics
@@ -137,6 +143,10 @@ struct gcctb_ctx {
   gcctb::DevBuf<uint32_t> sk0, sv0, sk1, sv1, flags, gstart, sort_tmp, S, SS;
   gcctb::DevBuf<uint64_t> sk64a, sk64b;
   gcctb::DevBuf<uint32_t> rank, tx_map, max_rank;
+  // hot-writer deferral: sampled counts and state, slot -> transaction map;
+  // in use while defer_active
+  gcctb::DevBuf<uint32_t> defer_cnt, defer_map;
+  bool defer_active = false;
   gcctb::DevBuf<uint64_t> conflicts;
   // watchdog diagnostics (GCCTB_DEBUG_DUMP=<file>)
   gcctb::DevBuf<uint64_t> dbg;
@@ -247,7 +257,7 @@ class SchemeHost {
     a.index = ctx.ix;
     a.ops = ctx.ops.p;
     a.tx_off = ctx.tx_off.p;
-    a.tx_map = nullptr;
+    a.tx_map = ctx.defer_active ? ctx.defer_map.p : nullptr;
     a.slot_lo = slot_lo;
     a.slot_hi = slot_hi;
     a.cc = ctx.cc.p;
@@ -691,6 +701,10 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
       const int t = std::atoi(pt);
       c->poll_ticks = (uint32_t)(t < 1 ? 1 : t > 1000 ? 1000 : t);
     }
+    if (const char* ds = std::getenv("GCCTB_DEFER_SHARE")) {
+      const int d = std::atoi(ds);
+      c->defer_share = d < 0 ? 0 : d > 65536 ? 65536 : d;
+    }
     const char* dump = std::getenv("GCCTB_DEBUG_DUMP");
     if (dump && *dump) c->dbg_path = dump;
   }
@@ -921,6 +935,10 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
       // an unpublished node from ever holding a link that points anywhere
       HIP_TRY(hipMemsetAsync(c->pool.p, 0, c->pool.n * sizeof(uint64_t), c->stream));
     }
+    if (c->scheme != SCHEME_GACCO && c->scheme != SCHEME_GPUTX && ntx) {
+      c->defer_cnt.alloc(launch::DEFER_WORDS);
+      c->defer_map.alloc(ntx);
+    }
     if (c->scheme == SCHEME_GACCO || c->scheme == SCHEME_GPUTX) {
       // sort buffers for the whole load (the persistent form prepares it at once)
       const uint64_t n = std::max(nops, ntx);
@@ -1030,8 +1048,17 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     double pre = 0, proc = 0;
     uint64_t launches = 0;
     HIP_TRY(hipEventRecord(c->ev[0], s));
+    // hot-writer deferral (lock, timestamp and OCC schemes; the ordered
+    // schemes wait on lower slots and keep the stream order)
+    const uint32_t share = c->defer_share >= 0                                          ? (uint32_t)c->defer_share
+                           : (c->scheme == SCHEME_SILO || c->scheme == SCHEME_TICTOC) ? 0u
+                                                                                        : 16u;
+    const bool defer = form == FORM_PERSIST && share && c->defer_map.p && c->scheme != SCHEME_GACCO &&
+                       c->scheme != SCHEME_GPUTX;
     if (form == FORM_PERSIST) {
       c->host->PrepStream();
+      if (defer)
+        HIP_TRY(launch::defer_order(c->tx_off.p, c->ops.p, c->ntx, c->nops, share, c->defer_cnt.p, c->defer_map.p, s));
       // Rows: GaccO's and GPUTx's preparation already has them; for the
       // other schemes a probe pass finds every op's row first, or each group
       // looks its rows up as it takes a transaction (WantsProbePass).
@@ -1042,7 +1069,9 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       HIP_TRY(launch::persist_begin(c->ctrs.p, s));
       HIP_TRY(hipEventRecord(c->ev[3], s));
       c->rows_probed = probed;
+      c->defer_active = defer;
       c->host->Launch(0, 0, c->ntx, true);
+      c->defer_active = false;
       c->rows_probed = false;
       HIP_TRY(hipEventRecord(c->ev[4], s));
       c->host->PostStream();
@@ -1141,6 +1170,11 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       st.ranks = c->nranks;
       if (c->nops) HIP_TRY(hipMemcpy(&st.same_rank_conflicts, c->conflicts.p, 8, hipMemcpyDeviceToHost));
     }
+    if (defer && c->ntx) {
+      uint32_t dst[3] = {0, 0, 0};
+      HIP_TRY(hipMemcpy(dst, c->defer_cnt.p + launch::DEFER_BUCKETS, sizeof dst, hipMemcpyDeviceToHost));
+      st.deferred = dst[launch::DEFER_ANY] ? dst[launch::DEFER_NHOT] : 0;
+    }
     int rate = 0;
     HIP_TRY(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0));
     st.tick_khz = rate;
@@ -1164,6 +1198,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
     }
   } catch (const HipFailure& f) {
     c->rows_probed = false;
+    c->defer_active = false;
     return hip_fail(f);
   }
   st.txs = c->ntx;

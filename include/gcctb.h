@@ -78,6 +78,8 @@ typedef struct gcctb_stats {
   uint64_t error_flags;    /* kernel error bits (0 = none) */
   uint64_t total_ops;      /* ops executed */
   uint64_t read_ops;       /* read ops executed */
+  uint64_t deferred;       /* persistent form of the lock, timestamp and OCC schemes: transactions
+                              admitted after all others because they write a hot key */
 } gcctb_stats;
 
 /* Create a context: replaces constructing the scheme by name and its device
