@@ -59,6 +59,7 @@ class Stats(C.Structure):
         ("index_miss", C.c_uint64), ("error_flags", C.c_uint64), ("total_ops", C.c_uint64),
         ("read_ops", C.c_uint64),
         ("deferred", C.c_uint64),
+        ("combined", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

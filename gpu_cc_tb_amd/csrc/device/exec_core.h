@@ -1275,6 +1275,10 @@ struct Ordered {
     const uint64_t now = now_ticks();
     if (c.ph == PH_START) {  // c.st[k] holds each op's turn (load_tx)
       begin_attempt(c, now);
+      // an op on a row that a combiner ran (GaccO, persistent form) is done
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (c.st[k] == NEED_DONE) c.meta[k] |= M_DONE;
       c.ph = PH_OPS;
     }
     if (now < c.t_resume) return;  // resting: no op can have its turn yet

@@ -78,8 +78,20 @@ hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t
                        uint64_t nrows, hipStream_t s);
 // need[op] = gstart[p] - segstart[p] for op = sop[p] & OPID_MASK: the turn
 // relative to the row's first access, for counters that start at zero.
+// With comb_min > 0 it also lists every row queue of at least comb_min
+// accesses as (first position, length) in segs (seg_ctr counts them; at most
+// seg_cap are kept).
 hipError_t access_need_rel(const uint32_t* sop, const uint32_t* gstart, const uint32_t* segstart, uint64_t n,
-                           uint32_t* need, hipStream_t s);
+                           uint32_t* need, hipStream_t s, const uint32_t* srow = nullptr, uint64_t nrows = 0,
+                           uint32_t comb_min = 0, uint2* segs = nullptr, uint32_t* seg_ctr = nullptr,
+                           uint32_t seg_cap = 0);
+// GaccO, persistent form: run each listed row queue in order on the row's
+// cells (cols <= 64), store its reads' results, mark its ops need = NEED_DONE
+// and add the ops run to *combined.
+hipError_t gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t seg_cap, const uint32_t* srow,
+                         const uint32_t* sop, const uint64_t* ops, const uint32_t* op_tx, const uint64_t* tx_off,
+                         uint64_t* cells, uint32_t cols, uint64_t seed, uint64_t* reads, uint32_t* need,
+                         uint64_t* combined, hipStream_t s);
 // Open form, one admitted batch sorted by (row, op) with group starts gstart
 // and row starts rstart: need[op] = base[row] + gstart[p] - rstart[p]; then
 // row_base_add adds the batch's accesses of each row to base[row].

@@ -532,10 +532,114 @@ __global__ void k_access_need(const uint32_t* srow, const uint32_t* sop, const u
 }
 
 __global__ void k_access_need_rel(const uint32_t* sop, const uint32_t* gstart, const uint32_t* segstart,
-                                  uint64_t n, uint32_t* need) {
+                                  uint64_t n, uint32_t* need, const uint32_t* srow, uint64_t nrows,
+                                  uint32_t comb_min, uint2* segs, uint32_t* seg_ctr, uint32_t seg_cap) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (uint64_t)gridDim.x * blockDim.x)
+       p += (uint64_t)gridDim.x * blockDim.x) {
     need[sop[p] & OPID_MASK] = gstart[p] - segstart[p];
+    // the last access of a row queue at least comb_min long lists the queue
+    // for a combiner
+    if (comb_min && (p + 1 == n || segstart[p + 1] != segstart[p])) {
+      const uint32_t len = (uint32_t)p + 1 - segstart[p];
+      if (len >= comb_min && srow[p] < nrows) {
+        const uint32_t i = atomicAdd(seg_ctr, 1u);
+        if (i < seg_cap) segs[i] = make_uint2(segstart[p], len);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GaccO, persistent form: a row whose access queue (its accesses in
+// transaction-id order) holds at least comb_min accesses is run by one
+// workgroup, the row's combiner, before the launch.  The combiner keeps the
+// row's cells in LDS and applies the queue in order, so every read returns
+// the value of the last write to its column before it in transaction-id
+// order, or the cell's value before the stream, and the row ends with the
+// last write of each column: what the queue's turns produce one handoff at a
+// time.  Its ops are marked done (need = NEED_DONE) and the launch skips
+// them; no other op touches the row.  Column mode, cols <= 64.
+//
+// Per step of CB_BT queue entries: every thread gathers one entry (column,
+// write flag, written value); wave 0 walks the step 64 entries at a time,
+// resolving each read against the writes below it in the same 64 (ballot
+// per written column) or the carried cell; then every thread stores its
+// read's result and the done mark.
+constexpr int CB_BT = 1024;
+__global__ __launch_bounds__(CB_BT) void k_gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t seg_cap,
+                                                         const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
+                                                         const uint32_t* op_tx, const uint64_t* tx_off, uint64_t* cells,
+                                                         uint32_t cols, uint64_t seed, uint64_t* reads, uint32_t* need,
+                                                         uint64_t* combined) {
+  __shared__ uint64_t val[CB_BT];   // the written value (writes), then the result (reads)
+  __shared__ uint32_t emeta[CB_BT]; // column | write << 8 | valid << 9
+  __shared__ uint64_t cell[64];
+  constexpr uint32_t E_WRITE = 0x100u, E_VALID = 0x200u;
+  const uint32_t nseg = min(*seg_ctr, seg_cap);
+  const uint32_t tid = threadIdx.x;
+  uint64_t done_ops = 0;
+  for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+    const uint2 seg = segs[sg];
+    const uint32_t p0 = seg.x, len = seg.y;
+    uint64_t* rc = cells + (uint64_t)srow[p0] * cols;
+    if (tid < cols) cell[tid] = rc[tid];
+    for (uint32_t b = 0; b < len; b += CB_BT) {
+      const uint32_t i = b + tid;
+      uint32_t opid = 0, em = 0;
+      if (i < len) {
+        const uint32_t so = sop[p0 + i];
+        opid = so & OPID_MASK;
+        const bool w = (so & OPID_WRITE) != 0;
+        em = op_col(ops[opid]) | (w ? E_WRITE : 0u) | E_VALID;
+        if (w) {
+          const uint32_t t = op_tx[opid];
+          val[tid] = write_value(seed, t, opid - (uint32_t)tx_off[t]);
+        }
+      }
+      emeta[tid] = em;
+      __syncthreads();
+      if (tid < 64) {
+        const uint32_t nstep = min(len - b, (uint32_t)CB_BT);
+        for (uint32_t e0 = 0; e0 < nstep; e0 += 64) {
+          const uint32_t e = e0 + tid;
+          const uint32_t me = e < nstep ? emeta[e] : 0u;
+          const bool valid = (me & E_VALID) != 0, isw = (me & E_WRITE) != 0;
+          const uint32_t col = me & 0xffu;
+          const uint64_t wv = isw ? val[e] : 0ull;
+          uint64_t rv = 0;
+          bool resolved = !valid || isw;
+          uint64_t wm = dev::ballot(valid && isw);
+          while (wm) {  // the columns written in these 64 entries, one at a time
+            const uint32_t c = dev::shfl32(col, __ffsll((unsigned long long)wm) - 1);
+            const uint64_t wc = dev::ballot(valid && isw && col == c);
+            const bool mine = valid && !isw && col == c;
+            const uint64_t below = wc & dev::lanemask_lt();
+            const int src = below ? 63 - __clzll((long long)below) : 0;
+            const uint64_t got = dev::shfl64(wv, src);
+            if (mine) {
+              rv = below ? got : cell[c];
+              resolved = true;
+            }
+            const uint64_t last = dev::shfl64(wv, 63 - __clzll((long long)wc));
+            if (tid == 0) cell[c] = last;
+            wm &= ~wc;
+          }
+          if (!resolved) rv = cell[col];  // a column nobody wrote in these 64
+          if (valid && !isw) val[e] = rv;
+        }
+      }
+      __syncthreads();
+      if (i < len) {
+        if (!(emeta[tid] & E_WRITE)) reads[opid] = val[tid];
+        need[opid] = NEED_DONE;
+      }
+      __syncthreads();  // val / emeta are rewritten by the next step
+    }
+    if (tid < cols) rc[tid] = cell[tid];
+    done_ops += len;
+    __syncthreads();  // cell is reloaded for the next row
+  }
+  if (tid == 0 && done_ops) atomicAdd((unsigned long long*)combined, (unsigned long long)done_ops);
 }
 
 // Open form (serving), one admitted batch at a time, accesses sorted by
@@ -861,8 +965,19 @@ hipError_t access_need(const uint32_t* srow, const uint32_t* sop, const uint32_t
   GC_LAUNCH(k_access_need, n, srow, sop, gstart, n, pos_base, need, counters, nrows);
 }
 hipError_t access_need_rel(const uint32_t* sop, const uint32_t* gstart, const uint32_t* segstart, uint64_t n,
-                           uint32_t* need, hipStream_t s) {
-  GC_LAUNCH(k_access_need_rel, n, sop, gstart, segstart, n, need);
+                           uint32_t* need, hipStream_t s, const uint32_t* srow, uint64_t nrows, uint32_t comb_min,
+                           uint2* segs, uint32_t* seg_ctr, uint32_t seg_cap) {
+  GC_LAUNCH(k_access_need_rel, n, sop, gstart, segstart, n, need, srow, nrows, comb_min, segs, seg_ctr, seg_cap);
+}
+hipError_t gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t seg_cap, const uint32_t* srow,
+                         const uint32_t* sop, const uint64_t* ops, const uint32_t* op_tx, const uint64_t* tx_off,
+                         uint64_t* cells, uint32_t cols, uint64_t seed, uint64_t* reads, uint32_t* need,
+                         uint64_t* combined, hipStream_t s) {
+  if (seg_cap == 0 || cols == 0 || cols > 64) return hipSuccess;
+  const uint32_t grid = seg_cap < 1024 ? seg_cap : 1024;
+  hipLaunchKernelGGL(k_gacco_combine, dim3(grid), dim3(CB_BT), 0, s, segs, seg_ctr, seg_cap, srow, sop, ops, op_tx,
+                     tx_off, cells, cols, seed, reads, need, combined);
+  return hipGetLastError();
 }
 hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,
                          hipStream_t s) {

@@ -53,6 +53,8 @@ GC_HD bool op_is_write(uint64_t op) { return (op >> 40) & 1u; }
 // bit 31: the value the GaccO / GPUTx preparation sorts carry.
 constexpr uint32_t OPID_WRITE = 0x80000000u;
 constexpr uint32_t OPID_MASK = 0x7fffffffu;
+// need[op] of an op that a row's combiner already executed (GaccO, persistent form)
+constexpr uint32_t NEED_DONE = 0xffffffffu;
 
 // ---------------------------------------------------------------------------
 // Deterministic hashing.  splitmix64 (Steele, Lea, Flood: "Fast splittable

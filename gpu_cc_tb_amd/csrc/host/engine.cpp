@@ -109,6 +109,9 @@ struct gcctb_ctx {
   // off); by default (-1) 16 for 2PL, T/O and MVCC, and off for Silo and
   // TicToc, which measured no gain from it
   int defer_share = -1;
+  // GaccO, persistent form: a row whose access queue holds at least this many
+  // accesses is run by a combiner before the launch (GCCTB_COMBINE_MIN; 0 off)
+  uint32_t comb_min = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   // pinned host word: the error flags of the launches since the last metrics
@@ -147,6 +150,10 @@ struct gcctb_ctx {
   // in use while defer_active
   gcctb::DevBuf<uint32_t> defer_cnt, defer_map;
   bool defer_active = false;
+  // GaccO combiners: queue list (first position, length) and counters
+  // ([0] low half: queues listed; [1]: ops the combiners ran)
+  gcctb::DevBuf<uint64_t> comb_segs, comb_ctr;
+  uint64_t comb_cap = 0;
   gcctb::DevBuf<uint64_t> conflicts;
   // watchdog diagnostics (GCCTB_DEBUG_DUMP=<file>)
   gcctb::DevBuf<uint64_t> dbg;
@@ -245,6 +252,8 @@ class SchemeHost {
   }
   // Device allocations of the scheme (GetMemSize, include/cc.cuh:39).
   virtual uint64_t GetMemSize() const { return ctx.cc.n * 8; }
+  // ops that row combiners ran in the last persistent pass (GaccO)
+  virtual uint64_t Combined() const { return 0; }
   std::vector<uint64_t> batches;
 
  protected:
@@ -431,7 +440,19 @@ class GaccoHost : public SchemeHost {
       HIP_TRY(launch::access_group_flags(srow, sop, n, ctx.flags.p, s, rflags));
       HIP_TRY(launch::inclusive_max_u32(ctx.flags.p, ctx.gstart.p, n, ctx.sort_tmp.p, s));
       HIP_TRY(launch::inclusive_max_u32(rflags, rstart, n, ctx.sort_tmp.p, s));
-      HIP_TRY(launch::access_need_rel(sop, ctx.gstart.p, rstart, n, ctx.need.p, s));
+      // rows with long queues go to combiners (whole stream, column mode)
+      const bool comb = ctx.scheme == SCHEME_GACCO && ctx.comb_min && ctx.comb_cap && ctx.cfg.cols <= 64 &&
+                        o0 == 0 && !ctx.cfg.row_ops && !ctx.tpcc;
+      if (comb) HIP_TRY(launch::fill_u64(ctx.comb_ctr.p, 2, 0, s));
+      uint2* segs = reinterpret_cast<uint2*>(ctx.comb_segs.p);
+      uint32_t* nseg = reinterpret_cast<uint32_t*>(ctx.comb_ctr.p);
+      HIP_TRY(launch::access_need_rel(sop, ctx.gstart.p, rstart, n, ctx.need.p, s, srow, ctx.cfg.rows,
+                                      comb ? ctx.comb_min : 0u, segs, nseg, (uint32_t)ctx.comb_cap));
+      if (comb)
+        HIP_TRY(launch::gacco_combine(segs, nseg, (uint32_t)ctx.comb_cap, srow, sop, ctx.ops.p, ctx.op_tx.p,
+                                      ctx.tx_off.p, ctx.cells.p, ctx.cfg.cols, ctx.cfg.seed, ctx.reads.p, ctx.need.p,
+                                      ctx.comb_ctr.p + 1, s));
+      combined_ = comb;
       HIP_TRY(launch::fill_u32(ctx.counters.p, ctx.counters.n, 0, s));
       return;
     }
@@ -446,6 +467,13 @@ class GaccoHost : public SchemeHost {
   }
   void Post(int, uint64_t, uint64_t) override {}
   uint64_t GetMemSize() const override { return ctx.counters.n * 4 + ctx.need.n * 4; }
+  // ops run by the combiners in the last persistent pass
+  uint64_t Combined() const override {
+    uint64_t v = 0;
+    if (combined_) HIP_TRY(hipMemcpy(&v, ctx.comb_ctr.p + 1, 8, hipMemcpyDeviceToHost));
+    return v;
+  }
+  bool combined_ = false;
 };
 
 // GPUTx: one-shot ranking of every transaction (GPUTx_CPU::Init0,
@@ -705,6 +733,10 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
       const int d = std::atoi(ds);
       c->defer_share = d < 0 ? 0 : d > 65536 ? 65536 : d;
     }
+    if (const char* cm = std::getenv("GCCTB_COMBINE_MIN")) {
+      const long v = std::atol(cm);
+      c->comb_min = (uint32_t)(v < 0 ? 0 : v > 0x7fffffffL ? 0x7fffffffL : v);
+    }
     const char* dump = std::getenv("GCCTB_DEBUG_DUMP");
     if (dump && *dump) c->dbg_path = dump;
   }
@@ -950,6 +982,12 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
       c->flags.alloc(n);
       c->gstart.alloc(n);
       c->sort_tmp.alloc(std::max(launch::radix_sort_tmp_words(n), launch::scan_tmp_words(n)));
+      if (c->scheme == SCHEME_GACCO && c->comb_min) {
+        // at most nops / comb_min queues reach comb_min accesses
+        c->comb_cap = std::min<uint64_t>(nops / c->comb_min + 1, 0x7fffffffull);
+        c->comb_segs.alloc(c->comb_cap);
+        c->comb_ctr.alloc(2);
+      }
       if (c->scheme == SCHEME_GPUTX) {
         c->S.alloc(n);
         c->SS.alloc(n);
@@ -1170,6 +1208,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       st.ranks = c->nranks;
       if (c->nops) HIP_TRY(hipMemcpy(&st.same_rank_conflicts, c->conflicts.p, 8, hipMemcpyDeviceToHost));
     }
+    if (form == FORM_PERSIST) st.combined = c->host->Combined();
     if (defer && c->ntx) {
       uint32_t dst[3] = {0, 0, 0};
       HIP_TRY(hipMemcpy(dst, c->defer_cnt.p + launch::DEFER_BUCKETS, sizeof dst, hipMemcpyDeviceToHost));
