@@ -560,80 +560,118 @@ __global__ void k_access_need_rel(const uint32_t* sop, const uint32_t* gstart, c
 // time.  Its ops are marked done (need = NEED_DONE) and the launch skips
 // them; no other op touches the row.  Column mode, cols <= 64.
 //
-// Per step of CB_BT queue entries: every thread gathers one entry (column,
-// write flag, written value); wave 0 walks the step 64 entries at a time,
-// resolving each read against the writes below it in the same 64 (ballot
-// per written column) or the carried cell; then every thread stores its
-// read's result and the done mark.
+// Steps of CB_STEP queue entries, double-buffered: while wave 0 walks step k,
+// the other waves gather step k + 1 (column, write flag, written value; three
+// dependent loads).  Wave 0 walks a step 64 entries at a time: per column
+// written among the 64 (a ballot each), every read of that column finds the
+// last write below it; then each read takes that write's value, or the
+// carried cell, with one LDS load, and the last write of each column
+// replaces the carried cell with one LDS store.  Finally every thread stores
+// its entry's read result and done mark.
 constexpr int CB_BT = 1024;
+constexpr int CB_STEP = CB_BT - 64;  // entries per step: one per thread of waves 1..15
+__device__ __forceinline__ void cb_gather(uint32_t i, uint32_t len, uint32_t p0, const uint32_t* sop, const uint64_t* ops,
+                                          const uint32_t* op_tx, const uint64_t* tx_off, uint64_t seed, uint32_t& opid,
+                                          uint32_t& em, uint64_t& wv) {
+  constexpr uint32_t E_WRITE = 0x100u, E_VALID = 0x200u;
+  opid = 0;
+  em = 0;
+  wv = 0;
+  if (i < len) {
+    const uint32_t so = sop[p0 + i];
+    opid = so & OPID_MASK;
+    const bool w = (so & OPID_WRITE) != 0;
+    em = op_col(ops[opid]) | (w ? E_WRITE : 0u) | E_VALID;
+    if (w) {
+      const uint32_t t = op_tx[opid];
+      wv = write_value(seed, t, opid - (uint32_t)tx_off[t]);
+    }
+  }
+}
 __global__ __launch_bounds__(CB_BT) void k_gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t seg_cap,
                                                          const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
                                                          const uint32_t* op_tx, const uint64_t* tx_off, uint64_t* cells,
                                                          uint32_t cols, uint64_t seed, uint64_t* reads, uint32_t* need,
                                                          uint64_t* combined) {
-  __shared__ uint64_t val[CB_BT];   // the written value (writes), then the result (reads)
-  __shared__ uint32_t emeta[CB_BT]; // column | write << 8 | valid << 9
+  __shared__ uint64_t val[2][CB_STEP];   // the written value (writes), then the result (reads)
+  __shared__ uint32_t emeta[2][CB_STEP]; // column | write << 8 | valid << 9
+  __shared__ uint32_t eop[2][CB_STEP];   // op id
   __shared__ uint64_t cell[64];
   constexpr uint32_t E_WRITE = 0x100u, E_VALID = 0x200u;
   const uint32_t nseg = min(*seg_ctr, seg_cap);
   const uint32_t tid = threadIdx.x;
+  const bool walker = tid < 64;            // wave 0 walks the queue
+  const uint32_t g = tid - 64;             // waves 1..15: gather slot
   uint64_t done_ops = 0;
   for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
     const uint2 seg = segs[sg];
     const uint32_t p0 = seg.x, len = seg.y;
     uint64_t* rc = cells + (uint64_t)srow[p0] * cols;
     if (tid < cols) cell[tid] = rc[tid];
-    for (uint32_t b = 0; b < len; b += CB_BT) {
-      const uint32_t i = b + tid;
-      uint32_t opid = 0, em = 0;
-      if (i < len) {
-        const uint32_t so = sop[p0 + i];
-        opid = so & OPID_MASK;
-        const bool w = (so & OPID_WRITE) != 0;
-        em = op_col(ops[opid]) | (w ? E_WRITE : 0u) | E_VALID;
-        if (w) {
-          const uint32_t t = op_tx[opid];
-          val[tid] = write_value(seed, t, opid - (uint32_t)tx_off[t]);
-        }
-      }
-      emeta[tid] = em;
-      __syncthreads();
-      if (tid < 64) {
-        const uint32_t nstep = min(len - b, (uint32_t)CB_BT);
+    const uint32_t nsteps = (len + CB_STEP - 1) / CB_STEP;
+    // step 0 gathered before the loop
+    if (!walker) {
+      uint32_t opid, em;
+      uint64_t wv;
+      cb_gather(g, len, p0, sop, ops, op_tx, tx_off, seed, opid, em, wv);
+      val[0][g] = wv;
+      emeta[0][g] = em;
+      eop[0][g] = opid;
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < nsteps; ++k) {
+      const uint32_t cur = k & 1, b = k * CB_STEP;
+      if (walker) {
+        const uint32_t nstep = min(len - b, (uint32_t)CB_STEP);
         for (uint32_t e0 = 0; e0 < nstep; e0 += 64) {
           const uint32_t e = e0 + tid;
-          const uint32_t me = e < nstep ? emeta[e] : 0u;
+          const uint32_t me = e < nstep ? emeta[cur][e] : 0u;
           const bool valid = (me & E_VALID) != 0, isw = (me & E_WRITE) != 0;
           const uint32_t col = me & 0xffu;
-          const uint64_t wv = isw ? val[e] : 0ull;
-          uint64_t rv = 0;
-          bool resolved = !valid || isw;
+          int src = -1;        // the entry (of these 64) whose write this read sees
+          bool last = false;   // the last write of its column among these 64
           uint64_t wm = dev::ballot(valid && isw);
-          while (wm) {  // the columns written in these 64 entries, one at a time
-            const uint32_t c = dev::shfl32(col, __ffsll((unsigned long long)wm) - 1);
+          while (wm) {
+            // the column of the lowest pending write (a scalar read of that lane)
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)col, __ffsll((unsigned long long)wm) - 1);
             const uint64_t wc = dev::ballot(valid && isw && col == c);
-            const bool mine = valid && !isw && col == c;
-            const uint64_t below = wc & dev::lanemask_lt();
-            const int src = below ? 63 - __clzll((long long)below) : 0;
-            const uint64_t got = dev::shfl64(wv, src);
-            if (mine) {
-              rv = below ? got : cell[c];
-              resolved = true;
+            if (valid && col == c) {
+              if (isw) {
+                last = (wc >> tid) >> 1 == 0;
+              } else {
+                const uint64_t below = wc & dev::lanemask_lt();
+                if (below) src = 63 - __clzll((long long)below);
+              }
             }
-            const uint64_t last = dev::shfl64(wv, 63 - __clzll((long long)wc));
-            if (tid == 0) cell[c] = last;
             wm &= ~wc;
           }
-          if (!resolved) rv = cell[col];  // a column nobody wrote in these 64
-          if (valid && !isw) val[e] = rv;
+          // every read first, then the carried cells move on
+          uint64_t rv = 0;
+          if (valid && !isw) rv = src >= 0 ? val[cur][e0 + (uint32_t)src] : cell[col];
+          if (valid && !isw) val[cur][e] = rv;
+          // a read and a last write are never the same lane, so nothing in one
+          // lane's code orders the carried-cell loads above before the stores
+          // below: keep the compiler from moving them (the wave's LDS accesses
+          // then run in program order)
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          if (last) cell[col] = val[cur][e];
         }
+      } else if (k + 1 < nsteps) {
+        uint32_t opid, em;
+        uint64_t wv;
+        cb_gather(b + CB_STEP + g, len, p0, sop, ops, op_tx, tx_off, seed, opid, em, wv);
+        val[cur ^ 1][g] = wv;
+        emeta[cur ^ 1][g] = em;
+        eop[cur ^ 1][g] = opid;
       }
       __syncthreads();
-      if (i < len) {
-        if (!(emeta[tid] & E_WRITE)) reads[opid] = val[tid];
+      // results of step k (waves 1..15, one entry each)
+      if (!walker && b + g < len) {
+        const uint32_t opid = eop[cur][g];
+        if (!(emeta[cur][g] & E_WRITE)) reads[opid] = val[cur][g];
         need[opid] = NEED_DONE;
       }
-      __syncthreads();  // val / emeta are rewritten by the next step
+      __syncthreads();  // buffer cur is gathered into again at step k + 2
     }
     if (tid < cols) rc[tid] = cell[tid];
     done_ops += len;
