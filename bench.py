@@ -203,7 +203,7 @@ def stream_leg(args, table, schemes):
         q = _one_pass(sc, args.rows, s.ops, shape, "plain", batch=1 << 30, batch_sizes=s.batch_sizes)
         pv, qv = s.ntx / (p["tot_ms"] / 1e3), s.ntx / (q["tot_ms"] / 1e3)
         out[sc] = {"persistent_txps": round(pv), "plain_txps": round(qv), "ratio": round(pv / qv, 3),
-                   "plain_launches": q["batches"], "wd": shape["warp_density"], "bs": shape["block_size"],
+                   "deferred": p["deferred"], "combined_ops": p["combined"], "plain_launches": q["batches"], "wd": shape["warp_density"], "bs": shape["block_size"],
                    "window": shape["window"]}
         _progress(f"stream {sc}: persistent {pv / 1e6:.2f} M, plain {qv / 1e6:.2f} M Tx/s ({pv / qv:.2f}x)")
     ratios = [v["ratio"] for v in out.values()]
@@ -217,7 +217,9 @@ def theta99_leg(args, table, schemes):
     shape against the plain form twice, per scheme, at read ratio 0.9 and 0.5:
     at the same shape with batch 65,536 (the reference's per-batch structure
     on the same launch shape), and at the plain form's own tuned shape and
-    batch size (best against best)."""
+    batch size (best against best).  The persistent pass includes its
+    hot-key handling, counted per scheme: transactions deferred behind the
+    others (2PL, T/O, MVCC) and ops run by GaccO's row combiners."""
     out = {}
     for rr in (0.9, 0.5):
         w = g.generate(args.rows, args.txs, theta=0.99, read_ratio=rr, len_min=16)
@@ -232,6 +234,7 @@ def theta99_leg(args, table, schemes):
             res[sc] = {"persistent_txps": round(pv), "plain_txps": round(qv), "ratio": round(pv / qv, 3),
                        "plain_tuned_txps": round(tv), "ratio_vs_tuned_plain": round(pv / tv, 3),
                        "persistent_aborts": p["aborts"], "plain_aborts": q["aborts"],
+                       "deferred": p["deferred"], "combined_ops": p["combined"],
                        "wd": shape["warp_density"], "bs": shape["block_size"], "window": shape["window"],
                        "plain_tuned": [pshape["warp_density"], pshape["block_size"], pshape["batch_size"]]}
             _progress(f"theta99 r{rr} {sc}: persistent {pv / 1e6:.2f} M, plain {qv / 1e6:.2f} M Tx/s ({pv / qv:.2f}x), "
