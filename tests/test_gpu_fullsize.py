@@ -125,6 +125,23 @@ def test_c3_high_contention(scheme, theta, rr, batch, table10m):
                   form="plain", what=f"C3 theta={theta} rr={rr}")
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("rr", [0.9, 0.5])
+@pytest.mark.parametrize("scheme", list(g.SCHEMES))
+def test_c3_persistent_theta99(scheme, rr, table10m):
+    """The persistent form at theta 0.99 and its tuned shape, as the bench's
+    theta 0.99 leg runs it: with the hot-writer deferral (2PL, T/O, MVCC) and
+    GaccO's row combiners engaged, replayed like every other run."""
+    w = workload(0.99, rr)
+    shape = autotune.lookup(scheme, 0.99, rr)
+    st = run_and_check(scheme, w, table10m, rows=ROWS, wd=shape["warp_density"], bs=shape["block_size"],
+                       batch=65536, form="persistent", window=shape["window"], what=f"C3 persistent rr={rr}")
+    if scheme in ("tpl_nw", "tpl_wd", "to", "mvcc"):
+        assert st["deferred"] > 0
+    if scheme == "gacco":
+        assert st["combined"] > 0
+
+
 @pytest.fixture(scope="module")
 def stream_c4():
     print("[fullsize] generating the 256-batch stream", flush=True)
