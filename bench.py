@@ -287,7 +287,7 @@ def serving(args, table):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path has no multi-GPU form in this build.",
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the library has no multi-GPU form.",
               file=sys.stderr)
         sys.exit(2)
     schemes = [s for s in args.schemes.split(",") if s]

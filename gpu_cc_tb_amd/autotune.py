@@ -45,7 +45,7 @@ PLAIN_WD = [1, 2, 3, 4, 5]
 PLAIN_BS = [4, 8, 16]
 PLAIN_BATCHES = [4096, 16384, 65536, 262144]
 # admission windows (transactions in flight; 0 = device-filling).  The
-# adaptive window (-1) is left out: the library refuses it in this build.
+# adaptive window (-1) is left out: the library refuses it.
 WINDOWS = [0, 32768, 16384, 8192, 4096, 1024, 512, 256]
 
 # starting shapes (from sweeps of the plain form at low contention)

@@ -689,12 +689,12 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   if (cfg->batch_size < 1) return set_error(GCCTB_EINVAL, "batch_size must be >= 1");
   if (cfg->rows < 1 || cfg->rows > 0xfffffffeull) return set_error(GCCTB_EINVAL, "rows out of range");
   if (cfg->cols < 1 || cfg->cols > 255) return set_error(GCCTB_EINVAL, "cols out of range");
-  // The adaptive admission window is not part of this build: with Silo it
+  // The adaptive admission window is not part of the library: with Silo it
   // once made a kernel access memory out of bounds (and once return reads
   // that differ from the serial replay), its cause was never shown, and its
   // code was taken out of the kernels.
   if (cfg->persist_adaptive)
-    return set_error(GCCTB_EINVAL, "persist_adaptive is not supported in this build; use a static persist_window");
+    return set_error(GCCTB_EINVAL, "persist_adaptive is not supported; use a static persist_window");
   if (cfg->row_ops && cfg->cols > (uint32_t)ROW_MAX_COLS)
     return set_error(GCCTB_EINVAL, "whole-row mode takes rows of at most 16 cells");
   int ndev = 0;

@@ -45,7 +45,7 @@ typedef struct gcctb_config {
                               is the failed attempt's duration when that is longer */
   uint32_t persist_window; /* persistent and serving forms: at most this many transactions in
                               flight (0 = as many as the device holds) */
-  int32_t persist_adaptive; /* not supported in this build: gcctb_create returns GCCTB_EINVAL
+  int32_t persist_adaptive; /* not supported: gcctb_create returns GCCTB_EINVAL
                                for 1 (the abort-rate-driven window was taken out) */
   int32_t row_ops;          /* 1: whole-row mode, the reference's Item copies
                                (benchmark/ycsb.cu:57,86-91): every op reads or writes all cols
