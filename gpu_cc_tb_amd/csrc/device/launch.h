@@ -87,11 +87,12 @@ hipError_t access_need_rel(const uint32_t* sop, const uint32_t* gstart, const ui
                            uint32_t seg_cap = 0);
 // GaccO, persistent form: run each listed row queue in order on the row's
 // cells (cols <= 64), store its reads' results, mark its ops need = NEED_DONE
-// and add the ops run to *combined.
+// and add the ops run to *combined.  wvals (nops u64) receives every op's
+// write value first, when any queue is listed.
 hipError_t gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t seg_cap, const uint32_t* srow,
                          const uint32_t* sop, const uint64_t* ops, const uint32_t* op_tx, const uint64_t* tx_off,
-                         uint64_t* cells, uint32_t cols, uint64_t seed, uint64_t* reads, uint32_t* need,
-                         uint64_t* combined, hipStream_t s);
+                         uint64_t nops, uint64_t* cells, uint32_t cols, uint64_t seed, uint64_t* wvals, uint64_t* reads,
+                         uint32_t* need, uint64_t* combined, hipStream_t s);
 // Open form, one admitted batch sorted by (row, op) with group starts gstart
 // and row starts rstart: need[op] = base[row] + gstart[p] - rstart[p]; then
 // row_base_add adds the batch's accesses of each row to base[row].

@@ -560,118 +560,94 @@ __global__ void k_access_need_rel(const uint32_t* sop, const uint32_t* gstart, c
 // time.  Its ops are marked done (need = NEED_DONE) and the launch skips
 // them; no other op touches the row.  Column mode, cols <= 64.
 //
-// Steps of CB_STEP queue entries, double-buffered: while wave 0 walks step k,
-// the other waves gather step k + 1 (column, write flag, written value; three
-// dependent loads).  Wave 0 walks a step 64 entries at a time: per column
-// written among the 64 (a ballot each), every read of that column finds the
-// last write below it; then each read takes that write's value, or the
-// carried cell, with one LDS load, and the last write of each column
-// replaces the carried cell with one LDS store.  Finally every thread stores
-// its entry's read result and done mark.
+// Steps of CB_BT queue entries, one per thread, all 16 waves at once:
+//  1. every thread gathers its entry (column, write flag, written value);
+//  2. each wave resolves its own 64 entries: per column written among them
+//     (one ballot each), a read finds the last write below it, and the last
+//     write of each column is kept as the wave's summary;
+//  3. lane c of wave 0 carries column c through the 16 summaries in order,
+//     which gives every wave the value each column holds at its start;
+//  4. the reads with no write below them in their wave take that value, and
+//     every thread stores its read's result and the done mark.
+// Written values come precomputed per op (k_gacco_wvals), so the gather is
+// two dependent loads deep.
 constexpr int CB_BT = 1024;
-constexpr int CB_STEP = CB_BT - 64;  // entries per step: one per thread of waves 1..15
-__device__ __forceinline__ void cb_gather(uint32_t i, uint32_t len, uint32_t p0, const uint32_t* sop, const uint64_t* ops,
-                                          const uint32_t* op_tx, const uint64_t* tx_off, uint64_t seed, uint32_t& opid,
-                                          uint32_t& em, uint64_t& wv) {
-  constexpr uint32_t E_WRITE = 0x100u, E_VALID = 0x200u;
-  opid = 0;
-  em = 0;
-  wv = 0;
-  if (i < len) {
-    const uint32_t so = sop[p0 + i];
-    opid = so & OPID_MASK;
-    const bool w = (so & OPID_WRITE) != 0;
-    em = op_col(ops[opid]) | (w ? E_WRITE : 0u) | E_VALID;
-    if (w) {
-      const uint32_t t = op_tx[opid];
-      wv = write_value(seed, t, opid - (uint32_t)tx_off[t]);
-    }
+constexpr int CB_WAVES = CB_BT / 64;
+// wv[op] = the value op would write, for every op (the combiners read those
+// of their writes); nothing to do when no queue is listed
+__global__ void k_gacco_wvals(const uint32_t* op_tx, const uint64_t* tx_off, uint64_t nops, uint64_t seed,
+                              const uint32_t* seg_ctr, uint64_t* wv) {
+  if (*seg_ctr == 0) return;  // the same for every thread of the grid
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nops;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t t = op_tx[i];
+    wv[i] = write_value(seed, t, (uint32_t)(i - tx_off[t]));
   }
 }
 __global__ __launch_bounds__(CB_BT) void k_gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t seg_cap,
                                                          const uint32_t* srow, const uint32_t* sop, const uint64_t* ops,
-                                                         const uint32_t* op_tx, const uint64_t* tx_off, uint64_t* cells,
-                                                         uint32_t cols, uint64_t seed, uint64_t* reads, uint32_t* need,
-                                                         uint64_t* combined) {
-  __shared__ uint64_t val[2][CB_STEP];   // the written value (writes), then the result (reads)
-  __shared__ uint32_t emeta[2][CB_STEP]; // column | write << 8 | valid << 9
-  __shared__ uint32_t eop[2][CB_STEP];   // op id
-  __shared__ uint64_t cell[64];
-  constexpr uint32_t E_WRITE = 0x100u, E_VALID = 0x200u;
+                                                         const uint64_t* wvals, uint64_t* cells, uint32_t cols,
+                                                         uint64_t* reads, uint32_t* need, uint64_t* combined) {
+  __shared__ uint64_t val[CB_BT];            // written values, by entry
+  __shared__ uint64_t sum[CB_WAVES][64];     // per wave: the last value it wrote to each column
+  __shared__ uint64_t cin[CB_WAVES][64];     // per wave: each column's value at its first entry
+  __shared__ uint64_t wmask[CB_WAVES];       // per wave: the columns it wrote
+  __shared__ uint64_t cell[64];              // the row between steps
   const uint32_t nseg = min(*seg_ctr, seg_cap);
-  const uint32_t tid = threadIdx.x;
-  const bool walker = tid < 64;            // wave 0 walks the queue
-  const uint32_t g = tid - 64;             // waves 1..15: gather slot
+  const uint32_t tid = threadIdx.x, w = tid / 64, lane = tid % 64;
   uint64_t done_ops = 0;
   for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
     const uint2 seg = segs[sg];
     const uint32_t p0 = seg.x, len = seg.y;
     uint64_t* rc = cells + (uint64_t)srow[p0] * cols;
     if (tid < cols) cell[tid] = rc[tid];
-    const uint32_t nsteps = (len + CB_STEP - 1) / CB_STEP;
-    // step 0 gathered before the loop
-    if (!walker) {
-      uint32_t opid, em;
-      uint64_t wv;
-      cb_gather(g, len, p0, sop, ops, op_tx, tx_off, seed, opid, em, wv);
-      val[0][g] = wv;
-      emeta[0][g] = em;
-      eop[0][g] = opid;
-    }
-    __syncthreads();
-    for (uint32_t k = 0; k < nsteps; ++k) {
-      const uint32_t cur = k & 1, b = k * CB_STEP;
-      if (walker) {
-        const uint32_t nstep = min(len - b, (uint32_t)CB_STEP);
-        for (uint32_t e0 = 0; e0 < nstep; e0 += 64) {
-          const uint32_t e = e0 + tid;
-          const uint32_t me = e < nstep ? emeta[cur][e] : 0u;
-          const bool valid = (me & E_VALID) != 0, isw = (me & E_WRITE) != 0;
-          const uint32_t col = me & 0xffu;
-          int src = -1;        // the entry (of these 64) whose write this read sees
-          bool last = false;   // the last write of its column among these 64
-          uint64_t wm = dev::ballot(valid && isw);
-          while (wm) {
-            // the column of the lowest pending write (a scalar read of that lane)
-            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)col, __ffsll((unsigned long long)wm) - 1);
-            const uint64_t wc = dev::ballot(valid && isw && col == c);
-            if (valid && col == c) {
-              if (isw) {
-                last = (wc >> tid) >> 1 == 0;
-              } else {
-                const uint64_t below = wc & dev::lanemask_lt();
-                if (below) src = 63 - __clzll((long long)below);
-              }
-            }
-            wm &= ~wc;
+    for (uint32_t b = 0; b < len; b += CB_BT) {
+      const uint32_t i = b + tid;
+      const bool valid = i < len;
+      uint32_t opid = 0, col = 0;
+      bool isw = false;
+      uint64_t wv = 0;
+      if (valid) {
+        const uint32_t so = sop[p0 + i];
+        opid = so & OPID_MASK;
+        isw = (so & OPID_WRITE) != 0;
+        col = op_col(ops[opid]);
+        if (isw) wv = wvals[opid];
+      }
+      int src = -1;  // the lane (of this wave) whose write this read sees
+      uint64_t written = 0;
+      uint64_t wm = dev::ballot(valid && isw);
+      while (wm) {  // the columns written among the wave's 64 entries
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)col, __ffsll((unsigned long long)wm) - 1);
+        const uint64_t wc = dev::ballot(valid && isw && col == c);
+        if (valid && col == c) {
+          if (isw) {
+            if ((wc >> lane) >> 1 == 0) sum[w][c] = wv;  // the wave's last write of c
+          } else {
+            const uint64_t below = wc & dev::lanemask_lt();
+            if (below) src = 63 - __clzll((long long)below);
           }
-          // every read first, then the carried cells move on
-          uint64_t rv = 0;
-          if (valid && !isw) rv = src >= 0 ? val[cur][e0 + (uint32_t)src] : cell[col];
-          if (valid && !isw) val[cur][e] = rv;
-          // a read and a last write are never the same lane, so nothing in one
-          // lane's code orders the carried-cell loads above before the stores
-          // below: keep the compiler from moving them (the wave's LDS accesses
-          // then run in program order)
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
-          if (last) cell[col] = val[cur][e];
         }
-      } else if (k + 1 < nsteps) {
-        uint32_t opid, em;
-        uint64_t wv;
-        cb_gather(b + CB_STEP + g, len, p0, sop, ops, op_tx, tx_off, seed, opid, em, wv);
-        val[cur ^ 1][g] = wv;
-        emeta[cur ^ 1][g] = em;
-        eop[cur ^ 1][g] = opid;
+        written |= 1ull << c;
+        wm &= ~wc;
+      }
+      val[tid] = wv;
+      if (lane == 0) wmask[w] = written;
+      __syncthreads();
+      if (tid < cols) {
+        uint64_t cur = cell[tid];
+        for (int x = 0; x < CB_WAVES; ++x) {
+          cin[x][tid] = cur;
+          if ((wmask[x] >> tid) & 1ull) cur = sum[x][tid];
+        }
+        cell[tid] = cur;
       }
       __syncthreads();
-      // results of step k (waves 1..15, one entry each)
-      if (!walker && b + g < len) {
-        const uint32_t opid = eop[cur][g];
-        if (!(emeta[cur][g] & E_WRITE)) reads[opid] = val[cur][g];
+      if (valid) {
+        if (!isw) reads[opid] = src >= 0 ? val[w * 64 + (uint32_t)src] : cin[w][col];
         need[opid] = NEED_DONE;
       }
-      __syncthreads();  // buffer cur is gathered into again at step k + 2
+      __syncthreads();  // val / sum / cin / wmask are rewritten by the next step
     }
     if (tid < cols) rc[tid] = cell[tid];
     done_ops += len;
@@ -1009,12 +985,16 @@ hipError_t access_need_rel(const uint32_t* sop, const uint32_t* gstart, const ui
 }
 hipError_t gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t seg_cap, const uint32_t* srow,
                          const uint32_t* sop, const uint64_t* ops, const uint32_t* op_tx, const uint64_t* tx_off,
-                         uint64_t* cells, uint32_t cols, uint64_t seed, uint64_t* reads, uint32_t* need,
-                         uint64_t* combined, hipStream_t s) {
-  if (seg_cap == 0 || cols == 0 || cols > 64) return hipSuccess;
+                         uint64_t nops, uint64_t* cells, uint32_t cols, uint64_t seed, uint64_t* wvals, uint64_t* reads,
+                         uint32_t* need, uint64_t* combined, hipStream_t s) {
+  if (seg_cap == 0 || cols == 0 || cols > 64 || nops == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gacco_wvals, dim3(grid_stride_blocks(nops)), dim3(BT), 0, s, op_tx, tx_off, nops, seed, seg_ctr,
+                     wvals);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   const uint32_t grid = seg_cap < 1024 ? seg_cap : 1024;
-  hipLaunchKernelGGL(k_gacco_combine, dim3(grid), dim3(CB_BT), 0, s, segs, seg_ctr, seg_cap, srow, sop, ops, op_tx,
-                     tx_off, cells, cols, seed, reads, need, combined);
+  hipLaunchKernelGGL(k_gacco_combine, dim3(grid), dim3(CB_BT), 0, s, segs, seg_ctr, seg_cap, srow, sop, ops, wvals,
+                     cells, cols, reads, need, combined);
   return hipGetLastError();
 }
 hipError_t gtx_conflicts(const uint64_t* skey, const uint32_t* flags, uint64_t n, uint64_t* conflicts,

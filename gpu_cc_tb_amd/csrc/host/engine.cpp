@@ -152,7 +152,7 @@ struct gcctb_ctx {
   bool defer_active = false;
   // GaccO combiners: queue list (first position, length) and counters
   // ([0] low half: queues listed; [1]: ops the combiners ran)
-  gcctb::DevBuf<uint64_t> comb_segs, comb_ctr;
+  gcctb::DevBuf<uint64_t> comb_segs, comb_ctr, comb_wv;
   uint64_t comb_cap = 0;
   gcctb::DevBuf<uint64_t> conflicts;
   // watchdog diagnostics (GCCTB_DEBUG_DUMP=<file>)
@@ -450,8 +450,8 @@ class GaccoHost : public SchemeHost {
                                       comb ? ctx.comb_min : 0u, segs, nseg, (uint32_t)ctx.comb_cap));
       if (comb)
         HIP_TRY(launch::gacco_combine(segs, nseg, (uint32_t)ctx.comb_cap, srow, sop, ctx.ops.p, ctx.op_tx.p,
-                                      ctx.tx_off.p, ctx.cells.p, ctx.cfg.cols, ctx.cfg.seed, ctx.reads.p, ctx.need.p,
-                                      ctx.comb_ctr.p + 1, s));
+                                      ctx.tx_off.p, ctx.nops, ctx.cells.p, ctx.cfg.cols, ctx.cfg.seed, ctx.comb_wv.p,
+                                      ctx.reads.p, ctx.need.p, ctx.comb_ctr.p + 1, s));
       combined_ = comb;
       HIP_TRY(launch::fill_u32(ctx.counters.p, ctx.counters.n, 0, s));
       return;
@@ -987,6 +987,7 @@ int gcctb_load_ops(gcctb_ctx* c, const uint64_t* ops, const uint64_t* tx_off, ui
         c->comb_cap = std::min<uint64_t>(nops / c->comb_min + 1, 0x7fffffffull);
         c->comb_segs.alloc(c->comb_cap);
         c->comb_ctr.alloc(2);
+        c->comb_wv.alloc(nops);  // every op's write value, for the combiners
       }
       if (c->scheme == SCHEME_GPUTX) {
         c->S.alloc(n);
