@@ -988,11 +988,12 @@ hipError_t gacco_combine(const uint2* segs, const uint32_t* seg_ctr, uint32_t se
                          uint64_t nops, uint64_t* cells, uint32_t cols, uint64_t seed, uint64_t* wvals, uint64_t* reads,
                          uint32_t* need, uint64_t* combined, hipStream_t s) {
   if (seg_cap == 0 || cols == 0 || cols > 64 || nops == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gacco_wvals, dim3(grid_stride_blocks(nops)), dim3(BT), 0, s, op_tx, tx_off, nops, seed, seg_ctr,
+  // a capped grid: with no queue listed every block leaves at once
+  hipLaunchKernelGGL(k_gacco_wvals, dim3(reduce_blocks(nops)), dim3(BT), 0, s, op_tx, tx_off, nops, seed, seg_ctr,
                      wvals);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint32_t grid = seg_cap < 1024 ? seg_cap : 1024;
+  const uint32_t grid = seg_cap < 512 ? seg_cap : 512;
   hipLaunchKernelGGL(k_gacco_combine, dim3(grid), dim3(CB_BT), 0, s, segs, seg_ctr, seg_cap, srow, sop, ops, wvals,
                      cells, cols, reads, need, combined);
   return hipGetLastError();
