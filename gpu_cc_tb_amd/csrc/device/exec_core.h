@@ -77,6 +77,7 @@ constexpr uint32_t M_VALID = 1u;   // op exists and its key was found in the ind
 constexpr uint32_t M_WRITE = 2u;
 constexpr uint32_t M_DONE = 4u;    // op finished in the current attempt
 constexpr uint32_t M_HELD = 8u;    // op holds a lock / uncommitted mark
+constexpr uint32_t M_HOT = 16u;    // op's key is hot (2PL locks it after the others)
 constexpr uint32_t M_COL_SHIFT = 8;
 
 // transaction phases
@@ -417,20 +418,28 @@ struct TwoPL {
         // First lock attempt of every unheld op at once: a CAS that guesses
         // a free word, all issued before any result is examined, so an
         // uncontended transaction locks in one atomic latency.
+        // Hot keys (M_HOT) are locked only once every other lock is held,
+        // so a hot lock is held for the commit alone.
+        bool cold = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) cold |= (c.meta[k] & (M_VALID | M_HELD | M_HOT)) == M_VALID;
+        const bool cold_left = group_any<G>(cold);
         uint64_t cur[K];
-        bool got[K];
+        bool got[K], tried[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const uint32_t m = c.meta[k];
           cur[k] = 0;
           got[k] = false;
-          if ((m & M_VALID) && !(m & M_HELD))
+          tried[k] = (m & M_VALID) && !(m & M_HELD) && !(cold_left && (m & M_HOT));
+          pending |= (m & M_VALID) && !(m & M_HELD) && !tried[k];
+          if (tried[k])
             got[k] = w_cas(cc_word(a, c.row[k]), cur[k], (m & M_WRITE) ? tpl::make_excl(c.tx) : tpl::CNT_ONE);
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const uint32_t m = c.meta[k];
-          if ((m & M_VALID) && !(m & M_HELD)) {
+          if (tried[k]) {
             uint64_t* w = cc_word(a, c.row[k]);
             StepResult r = R_OK;
             if (!got[k]) r = (m & M_WRITE) ? lock_excl(w, c.tx, cur[k]) : lock_shared(w, c.tx, cur[k]);
@@ -1434,6 +1443,7 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
     for (int k = 0; k < K; ++k)
       if (has[k]) a.rows_out[c.op0 + op_index<G>(k)] = rowv[k];
   }
+  const bool hot = a.hot_cnt && *a.hot_any;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     c.meta[k] = 0;
@@ -1446,6 +1456,7 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
       if (row < a.rows) {
         c.row[k] = row;
         c.meta[k] = M_VALID | (op_is_write(op) ? M_WRITE : 0u) | (op_col(op) << M_COL_SHIFT);
+        if (hot && a.hot_cnt[defer_bucket(op_key(op))] >= a.hot_thr) c.meta[k] |= M_HOT;
         if (a.need) c.st[k] = ld_in32(a.need + c.op0 + j, coh);  // ordered schemes: the op's turn on its row
       } else {
         c.m_miss++;

@@ -136,13 +136,14 @@ hipError_t microbench(int which, uint64_t n_words, uint64_t threads_total, uint6
 // DEFER_SAMPLE sampled ops).  cnt holds DEFER_WORDS u32: DEFER_BUCKETS
 // sampled counts, then the state words (cnt + DEFER_BUCKETS)[DEFER_ANY] (a
 // key was hot) and [DEFER_NHOT] (the number deferred); map holds ntx u32.
-constexpr uint32_t DEFER_BUCKET_BITS = 16;
 constexpr uint32_t DEFER_BUCKETS = 1u << DEFER_BUCKET_BITS;
 constexpr uint64_t DEFER_SAMPLE = 16384;
 constexpr uint32_t DEFER_ANY = 0, DEFER_NCOLD = 1, DEFER_NHOT = 2;
 constexpr uint64_t DEFER_WORDS = DEFER_BUCKETS + 8;
 hipError_t defer_order(const uint64_t* tx_off, const uint64_t* ops, uint64_t ntx, uint64_t nops, uint32_t share,
                        uint32_t* cnt, uint32_t* map, hipStream_t s);
+// the sampled-count threshold defer_order uses for a hot key
+uint32_t defer_threshold(uint64_t ntx, uint64_t nops, uint32_t share);
 
 // --- per-batch state ---
 // clear the CC words (and MVCC chain heads) of rows[0, n) at their slots

@@ -806,9 +806,6 @@ __global__ void k_clock_sample(uint64_t* out) {
 // stride-th op); thr is scaled so that a key touched by at least one
 // transaction in `share` qualifies.  The deferral only reorders admission:
 // each scheme still reports its own serialization order.
-__device__ __forceinline__ uint32_t defer_bucket(uint32_t key) {
-  return (key * 2654435761u) >> (32 - launch::DEFER_BUCKET_BITS);
-}
 
 // sampled access counts per bucket; the lanes of a wave that share a bucket
 // add together (one atomic per distinct bucket of the wave)
@@ -925,14 +922,18 @@ hipError_t op_ids(const uint64_t* ops, uint64_t n, uint32_t base, uint32_t* out,
   GC_LAUNCH(k_op_ids, n, ops, n, base, out);
 }
 hipError_t fill_u32(uint32_t* v, uint64_t n, uint32_t x, hipStream_t s) { GC_LAUNCH(k_fill32, n, v, n, x); }
+uint32_t defer_threshold(uint64_t ntx, uint64_t nops, uint32_t share) {
+  const uint64_t stride = nops > DEFER_SAMPLE ? nops / DEFER_SAMPLE : 1;
+  const uint64_t want = (ntx + (uint64_t)share * stride - 1) / ((uint64_t)share * stride);
+  return (uint32_t)(want < 2 ? 2 : want > 0xffffffffull ? 0xffffffffull : want);
+}
 hipError_t defer_order(const uint64_t* tx_off, const uint64_t* ops, uint64_t ntx, uint64_t nops, uint32_t share,
                        uint32_t* cnt, uint32_t* map, hipStream_t s) {
   if (ntx == 0) return hipSuccess;
   // about DEFER_SAMPLE sampled ops
   const uint64_t stride = nops > DEFER_SAMPLE ? nops / DEFER_SAMPLE : 1;
   const uint64_t nsamp = nops / stride;
-  const uint64_t want = (ntx + (uint64_t)share * stride - 1) / ((uint64_t)share * stride);
-  const uint32_t thr = (uint32_t)(want < 2 ? 2 : want > 0xffffffffull ? 0xffffffffull : want);
+  const uint32_t thr = defer_threshold(ntx, nops, share);
   uint32_t* st = cnt + DEFER_BUCKETS;
   hipLaunchKernelGGL(k_fill32, dim3(grid_stride_blocks(DEFER_WORDS)), dim3(BT), 0, s, cnt, (uint64_t)DEFER_WORDS, 0u);
   hipError_t e = hipGetLastError();

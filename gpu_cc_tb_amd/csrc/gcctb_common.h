@@ -53,6 +53,9 @@ GC_HD bool op_is_write(uint64_t op) { return (op >> 40) & 1u; }
 // bit 31: the value the GaccO / GPUTx preparation sorts carry.
 constexpr uint32_t OPID_WRITE = 0x80000000u;
 constexpr uint32_t OPID_MASK = 0x7fffffffu;
+// The key bucket of the hot-key counts (Fibonacci hash into 2^16 buckets).
+constexpr uint32_t DEFER_BUCKET_BITS = 16;
+GC_HD uint32_t defer_bucket(uint32_t key) { return (key * 2654435761u) >> (32 - DEFER_BUCKET_BITS); }
 // need[op] of an op that a row's combiner already executed (GaccO, persistent form)
 constexpr uint32_t NEED_DONE = 0xffffffffu;
 
@@ -272,6 +275,12 @@ struct ExecArgs {
   const uint64_t* ops;
   const uint64_t* tx_off;
   const uint32_t* tx_map;  // launch slot -> transaction id (null: identity)
+  // hot keys (2PL, persistent form with the deferral): sampled counts per
+  // key bucket, the hot threshold, and the word that says whether any key
+  // is hot (null: no hot keys)
+  const uint32_t* hot_cnt;
+  const uint32_t* hot_any;
+  uint32_t hot_thr;
   uint64_t slot_lo;        // launch covers slots [slot_lo, slot_hi)
   uint64_t slot_hi;
   // CC state

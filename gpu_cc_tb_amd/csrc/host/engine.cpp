@@ -111,7 +111,7 @@ struct gcctb_ctx {
   int defer_share = -1;
   // GaccO, persistent form: a row whose access queue holds at least this many
   // accesses is run by a combiner before the launch (GCCTB_COMBINE_MIN; 0 off)
-  uint32_t comb_min = 256;
+  uint32_t comb_min = 128;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   // pinned host word: the error flags of the launches since the last metrics
@@ -150,6 +150,9 @@ struct gcctb_ctx {
   // in use while defer_active
   gcctb::DevBuf<uint32_t> defer_cnt, defer_map;
   bool defer_active = false;
+  uint32_t defer_thr = 0;  // the pass's hot threshold (sampled accesses)
+  // 2PL: lock hot keys after the others (GCCTB_HOT_LAST=0 turns it off)
+  bool hot_last = true;
   // GaccO combiners: queue list (first position, length) and counters
   // ([0] low half: queues listed; [1]: ops the combiners ran)
   gcctb::DevBuf<uint64_t> comb_segs, comb_ctr, comb_wv;
@@ -267,6 +270,14 @@ class SchemeHost {
     a.ops = ctx.ops.p;
     a.tx_off = ctx.tx_off.p;
     a.tx_map = ctx.defer_active ? ctx.defer_map.p : nullptr;
+    a.hot_cnt = nullptr;
+    a.hot_any = nullptr;
+    a.hot_thr = 0;
+    if (ctx.defer_active && ctx.hot_last && (ctx.scheme == SCHEME_TPL_NW || ctx.scheme == SCHEME_TPL_WD)) {
+      a.hot_cnt = ctx.defer_cnt.p;
+      a.hot_any = ctx.defer_cnt.p + launch::DEFER_BUCKETS + launch::DEFER_ANY;
+      a.hot_thr = ctx.defer_thr;
+    }
     a.slot_lo = slot_lo;
     a.slot_hi = slot_hi;
     a.cc = ctx.cc.p;
@@ -733,6 +744,7 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
       const int d = std::atoi(ds);
       c->defer_share = d < 0 ? 0 : d > 65536 ? 65536 : d;
     }
+    if (const char* hl = std::getenv("GCCTB_HOT_LAST")) c->hot_last = std::atoi(hl) != 0;
     if (const char* cm = std::getenv("GCCTB_COMBINE_MIN")) {
       const long v = std::atol(cm);
       c->comb_min = (uint32_t)(v < 0 ? 0 : v > 0x7fffffffL ? 0x7fffffffL : v);
@@ -1098,6 +1110,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       c->host->PrepStream();
       if (defer)
         HIP_TRY(launch::defer_order(c->tx_off.p, c->ops.p, c->ntx, c->nops, share, c->defer_cnt.p, c->defer_map.p, s));
+      if (defer) c->defer_thr = launch::defer_threshold(c->ntx, c->nops, share);
       // Rows: GaccO's and GPUTx's preparation already has them; for the
       // other schemes a probe pass finds every op's row first, or each group
       // looks its rows up as it takes a transaction (WantsProbePass).
