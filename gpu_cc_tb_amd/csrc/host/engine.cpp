@@ -151,6 +151,7 @@ struct gcctb_ctx {
   gcctb::DevBuf<uint32_t> defer_cnt, defer_map;
   bool defer_active = false;
   uint32_t defer_thr = 0;  // the pass's hot threshold (sampled accesses)
+  bool hot_active = false; // the launch reads hot flags (2PL with the deferral)
   // 2PL: lock hot keys after the others (GCCTB_HOT_LAST=0 turns it off)
   bool hot_last = true;
   // GaccO combiners: queue list (first position, length) and counters
@@ -273,7 +274,7 @@ class SchemeHost {
     a.hot_cnt = nullptr;
     a.hot_any = nullptr;
     a.hot_thr = 0;
-    if (ctx.defer_active && ctx.hot_last && (ctx.scheme == SCHEME_TPL_NW || ctx.scheme == SCHEME_TPL_WD)) {
+    if (ctx.hot_active) {
       a.hot_cnt = ctx.defer_cnt.p;
       a.hot_any = ctx.defer_cnt.p + launch::DEFER_BUCKETS + launch::DEFER_ANY;
       a.hot_thr = ctx.defer_thr;
@@ -1106,11 +1107,14 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
                                                                                         : 16u;
     const bool defer = form == FORM_PERSIST && share && c->defer_map.p && c->scheme != SCHEME_GACCO &&
                        c->scheme != SCHEME_GPUTX;
+    // 2PL with the deferral also locks hot keys last
+    const bool hot = defer && c->hot_last && (c->scheme == SCHEME_TPL_NW || c->scheme == SCHEME_TPL_WD);
     if (form == FORM_PERSIST) {
       c->host->PrepStream();
-      if (defer)
+      if (defer) {
         HIP_TRY(launch::defer_order(c->tx_off.p, c->ops.p, c->ntx, c->nops, share, c->defer_cnt.p, c->defer_map.p, s));
-      if (defer) c->defer_thr = launch::defer_threshold(c->ntx, c->nops, share);
+        c->defer_thr = launch::defer_threshold(c->ntx, c->nops, share);
+      }
       // Rows: GaccO's and GPUTx's preparation already has them; for the
       // other schemes a probe pass finds every op's row first, or each group
       // looks its rows up as it takes a transaction (WantsProbePass).
@@ -1122,8 +1126,10 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
       HIP_TRY(hipEventRecord(c->ev[3], s));
       c->rows_probed = probed;
       c->defer_active = defer;
+      c->hot_active = hot;
       c->host->Launch(0, 0, c->ntx, true);
       c->defer_active = false;
+      c->hot_active = false;
       c->rows_probed = false;
       HIP_TRY(hipEventRecord(c->ev[4], s));
       c->host->PostStream();
@@ -1252,6 +1258,7 @@ static int run_form(gcctb_ctx* c, gcctb_stats* out, RunForm form) {
   } catch (const HipFailure& f) {
     c->rows_probed = false;
     c->defer_active = false;
+    c->hot_active = false;
     return hip_fail(f);
   }
   st.txs = c->ntx;

@@ -81,7 +81,7 @@ typedef struct gcctb_stats {
   uint64_t deferred;       /* persistent form of the lock, timestamp and OCC schemes: transactions
                               admitted after all others because they write a hot key */
   uint64_t combined;       /* persistent form of gacco: ops run by hot-row combiners (a row whose
-                              queue holds at least 256 accesses is run in queue order by one
+                              queue holds at least 128 accesses is run in queue order by one
                               workgroup before the launch) */
 } gcctb_stats;
 

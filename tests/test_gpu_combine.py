@@ -9,6 +9,7 @@ import pytest
 import gpu_cc_tb_amd as g
 import oracle_lib as orc
 
+COMBINE_MIN = 128  # the default threshold (GCCTB_COMBINE_MIN)
 CASES = {
     # name: (rows, ntx, theta, read_ratio, len_min, len_max, cols)
     "reads": (4096, 8192, 0.99, 0.9, 16, 16, 10),
@@ -35,8 +36,8 @@ def test_combiners_match_serial_and_turns(case, monkeypatch):
     w = g.generate(rows, ntx, theta=theta, read_ratio=rr, len_min=lmin, len_max=lmax, cols=cols)
     keys = g.op_key(w.ops).astype(np.int64)
     per_row = np.bincount(keys, minlength=rows)
-    expect = int(per_row[per_row >= 256].sum())
-    print(f"[combine] case={case} rows with >= 256 accesses: {(per_row >= 256).sum()}", flush=True)
+    expect = int(per_row[per_row >= COMBINE_MIN].sum())
+    print(f"[combine] case={case} rows with >= {COMBINE_MIN} accesses: {(per_row >= COMBINE_MIN).sum()}", flush=True)
     st, res = run(rows, cols, w)
     assert st["committed"] == ntx and st["error_flags"] == 0
     assert st["combined"] == expect and expect > 0
