@@ -35,10 +35,11 @@ hipError_t seed_table(uint64_t* cells, uint64_t rows, uint32_t cols, uint64_t se
 hipError_t index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, uint32_t* buckets,
                         uint32_t nbuckets, hipStream_t s);
 // rows_out[i] = row of ops[i]'s key; also into rows_copy when it is given;
-// ids_out[i] = what op_ids(ops, n, id_base) writes, when it is given
+// ids_out[i] = what op_ids(ops, n, id_base) writes, when it is given; with
+// id_map, id_map[i] with the op's write flag in bit 31 instead
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
                  hipStream_t s, uint32_t* rows_copy = nullptr, uint32_t* ids_out = nullptr,
-                 uint32_t id_base = 0);
+                 uint32_t id_base = 0, const uint32_t* id_map = nullptr);
 
 // --- per-op helpers ---
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s);
@@ -111,9 +112,11 @@ hipError_t gtx_conflicts32(const uint32_t* skey, const uint32_t* flags, uint64_t
 // starts else 0.
 hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint64_t nrows, uint32_t* boundary,
                      uint32_t* rowstart, hipStream_t s);
-// local rank = S[p] - S[SS[p]]; rank[tx] = max; max_rank = max.
-hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
-                    const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s);
+// local rank = S[p] - S[SS[p]]; rank[tx] = max over tx's accesses, where
+// stx[p] is the transaction at sorted position p (write flag in bit 31);
+// max_rank = max.
+hipError_t gtx_rank(const uint32_t* stx, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                    uint32_t* rank, uint32_t* max_rank, hipStream_t s);
 // composite key (row << rank_bits) | rank[op_tx[op]] for ops [0, n)
 hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                          uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s);

@@ -133,8 +133,11 @@ __device__ __forceinline__ bool quad_bucket_match(const uint4& w, uint32_t key, 
 // instruction covers one line per probe and the four buckets are read at
 // once.  A probe whose bucket is full without the key goes on bucket by
 // bucket, the quad together.
+// ids_out[i] (optional): the op's id (id_base + i), or id_map[i] when given,
+// with the op's write flag in bit 31
 __global__ __launch_bounds__(256) void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                                               uint32_t* rows_copy, uint32_t* ids_out, uint32_t id_base) {
+                                               uint32_t* rows_copy, uint32_t* ids_out, uint32_t id_base,
+                                               const uint32_t* id_map) {
   const uint32_t q = threadIdx.x & 3u;
   const uint4* bkt = reinterpret_cast<const uint4*>(ix.buckets);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of 4
@@ -174,7 +177,8 @@ __global__ __launch_bounds__(256) void k_probe(IndexView ix, const uint64_t* ops
     if (mine < n) {
       rows_out[mine] = r;
       if (rows_copy) rows_copy[mine] = r;
-      if (ids_out) ids_out[mine] = (id_base + (uint32_t)mine) | (op_is_write(myop) ? OPID_WRITE : 0u);
+      if (ids_out)
+        ids_out[mine] = (id_map ? id_map[mine] : id_base + (uint32_t)mine) | (op_is_write(myop) ? OPID_WRITE : 0u);
     }
   }
 }
@@ -708,13 +712,14 @@ __global__ void k_gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t 
                    ((sop[p] | sop[p - 1]) & OPID_WRITE) != 0u) ? 1u : 0u;
   }
 }
-__global__ void k_gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
-                           const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank) {
+// stx[p]: the transaction of sorted position p (write flag in bit 31)
+__global__ void k_gtx_rank(const uint32_t* stx, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                           uint32_t* rank, uint32_t* max_rank) {
   uint32_t mx = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t local = S[p] - S[SS[p]];
-    if (local) atomicMax(&rank[op_tx[sop[p] & OPID_MASK]], local);  // ranks start at 0
+    if (local) atomicMax(&rank[stx[p] & OPID_MASK], local);  // ranks start at 0
     mx = local > mx ? local : mx;
   }
   // one atomic per block (launched with reduce_blocks)
@@ -909,8 +914,8 @@ hipError_t index_insert(const uint32_t* keys, const uint32_t* vals, uint64_t n, 
   GC_LAUNCH(k_index_insert, n, keys, vals, n, buckets, nbuckets);
 }
 hipError_t probe(const IndexView& ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
-                 hipStream_t s, uint32_t* rows_copy, uint32_t* ids_out, uint32_t id_base) {
-  GC_LAUNCH(k_probe, n, ix, ops, n, rows_out, rows_copy, ids_out, id_base);
+                 hipStream_t s, uint32_t* rows_copy, uint32_t* ids_out, uint32_t id_base, const uint32_t* id_map) {
+  GC_LAUNCH(k_probe, n, ix, ops, n, rows_out, rows_copy, ids_out, id_base, id_map);
 }
 hipError_t op_tx(const uint64_t* tx_off, uint64_t ntx, uint32_t* op_tx, hipStream_t s) {
   GC_LAUNCH(k_op_tx, ntx, tx_off, ntx, op_tx);
@@ -1011,9 +1016,9 @@ hipError_t gtx_flags(const uint32_t* srow, const uint32_t* sop, uint64_t n, uint
                      uint32_t* rowstart, hipStream_t s) {
   GC_LAUNCH(k_gtx_flags, n, srow, sop, n, nrows, boundary, rowstart);
 }
-hipError_t gtx_rank(const uint32_t* sop, const uint32_t* S, const uint32_t* SS, uint64_t n,
-                    const uint32_t* op_tx, uint32_t* rank, uint32_t* max_rank, hipStream_t s) {
-  GC_LAUNCH_REDUCE(k_gtx_rank, n, sop, S, SS, n, op_tx, rank, max_rank);
+hipError_t gtx_rank(const uint32_t* stx, const uint32_t* S, const uint32_t* SS, uint64_t n,
+                    uint32_t* rank, uint32_t* max_rank, hipStream_t s) {
+  GC_LAUNCH_REDUCE(k_gtx_rank, n, stx, S, SS, n, rank, max_rank);
 }
 hipError_t gtx_exec_keys(const uint32_t* rows, const uint32_t* op_tx, const uint32_t* rank,
                          uint64_t n, int rank_bits, uint64_t* keys, hipStream_t s) {

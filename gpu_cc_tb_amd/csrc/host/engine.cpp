@@ -525,20 +525,22 @@ class GPUTxHost : public SchemeHost {
     ctx.rank_counts_ready = false;
     const uint64_t n = ctx.nops, ntx = ctx.ntx;
     hipStream_t s = ctx.stream;
-    // per-row access lists in (row, tx) order
-    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p, ctx.sv0.p, 0));
-    uint32_t *srow, *sop;
+    // per-row access lists in (row, tx) order; the sorted values are the
+    // accesses' transactions (with the write flag), so the rank pass needs
+    // no op -> transaction gather
+    HIP_TRY(launch::probe(ctx.ix, ctx.ops.p, n, ctx.rows_out.p, s, ctx.sk0.p, ctx.sv0.p, 0, ctx.op_tx.p));
+    uint32_t *srow, *stx_ops;
     // rows are below cfg.rows; a miss (IDX_MISS) has all row bits set and sorts last
     const int row_bits = bits_for(ctx.cfg.rows);
     HIP_TRY(launch::radix_sort_u32(ctx.sk0.p, ctx.sv0.p, ctx.sk1.p, ctx.sv1.p, n, row_bits, ctx.sort_tmp.p,
-                                   &srow, &sop, s));
+                                   &srow, &stx_ops, s));
     // one-pass local ranks and rank[tx] = max (src/gputx.cu:5-27)
-    HIP_TRY(launch::gtx_flags(srow, sop, n, ctx.cfg.rows, ctx.flags.p, ctx.gstart.p, s));
+    HIP_TRY(launch::gtx_flags(srow, stx_ops, n, ctx.cfg.rows, ctx.flags.p, ctx.gstart.p, s));
     HIP_TRY(launch::inclusive_sum_u32(ctx.flags.p, ctx.S.p, n, ctx.sort_tmp.p, s));
     HIP_TRY(launch::inclusive_max_u32(ctx.gstart.p, ctx.SS.p, n, ctx.sort_tmp.p, s));
     HIP_TRY(launch::fill_u32(ctx.rank.p, ntx, 0, s));
     HIP_TRY(launch::fill_u32(ctx.max_rank.p, 1, 0, s));
-    HIP_TRY(launch::gtx_rank(sop, ctx.S.p, ctx.SS.p, n, ctx.op_tx.p, ctx.rank.p, ctx.max_rank.p, s));
+    HIP_TRY(launch::gtx_rank(stx_ops, ctx.S.p, ctx.SS.p, n, ctx.rank.p, ctx.max_rank.p, s));
     uint32_t max_rank = 0;
     HIP_TRY(hipMemcpyAsync(&max_rank, ctx.max_rank.p, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
