@@ -38,8 +38,8 @@ TABLE_PATH = os.path.join(HERE, "tuned.json")
 THETAS = {"0": 0.1, "1": 0.6, "2": 0.8, "3": 0.9, "4": 0.99}
 READS = {"w": 0.5, "r": 0.9, "o": 1.0}
 
-WD = [1, 2, 3, 4]
-BS = [4, 8]
+WD = [1, 2, 3, 4, 5]
+BS = [1, 2, 4, 8]
 # plain form: lanes per transaction, block size (in 32-lane units) and batch size
 PLAIN_WD = [1, 2, 3, 4, 5]
 PLAIN_BS = [4, 8, 16]
