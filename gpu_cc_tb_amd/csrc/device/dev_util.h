@@ -196,6 +196,7 @@ __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 // IDX_MISS instead, so that it can never be mistaken for a row when the index
 // is shorter than the table (gcctb_probe maps it back to the length).
 __device__ __forceinline__ uint32_t index_find(const IndexView& ix, uint32_t key) {
+  if (ix.dense) return idx_dense_find(key, ix.length);
   if (key == IDX_EMPTY) return IDX_MISS;  // reserved, never a key
   uint32_t b = idx_bucket(key, ix.nbuckets);
   for (uint32_t step = 0; step < ix.nbuckets; ++step) {

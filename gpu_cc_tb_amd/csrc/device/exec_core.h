@@ -1392,9 +1392,9 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
     c.nops = (uint32_t)(ld_in(a.tx_off + c.tx + 1, coh) - c.op0);
   }
   // Every slot's op, then every slot's row, each step issued for all slots
-  // before its wait: rows from the probe pass, or from the hash index (all
-  // home buckets read at once; a key whose home bucket is full without it
-  // walks on through index_find).
+  // before its wait: rows from the probe pass, from the key itself (a dense
+  // index), or from the hash index (all home buckets read at once; a key
+  // whose home bucket is full without it walks on through index_find).
   uint64_t opv[K];
   uint32_t rowv[K];
   bool has[K];
@@ -1406,6 +1406,12 @@ __device__ __forceinline__ void load_tx(TxCtx<G>& c, const ExecArgs& a, uint64_t
   if (a.rows_in) {
 #pragma unroll
     for (int k = 0; k < K; ++k) rowv[k] = has[k] ? ld_in32(a.rows_in + c.op0 + op_index<G>(k), coh) : IDX_MISS;
+  } else if (a.index.dense) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      rowv[k] = has[k] ? idx_dense_find(op_key(opv[k]), a.index.length) : IDX_MISS;
+      if (has[k]) a.rows_out[c.op0 + op_index<G>(k)] = rowv[k];
+    }
   } else {
     uint4 q0[K], q1[K];
     const uint32_t* bk[K];

@@ -132,12 +132,23 @@ __device__ __forceinline__ bool quad_bucket_match(const uint4& w, uint32_t key, 
 // q reads the q-th 16 bytes of each probe's home bucket, so one load
 // instruction covers one line per probe and the four buckets are read at
 // once.  A probe whose bucket is full without the key goes on bucket by
-// bucket, the quad together.
+// bucket, the quad together.  A dense index (keys 0 .. length-1 on rows of
+// the same number) maps each key to itself and reads no bucket.
 // ids_out[i] (optional): the op's id (id_base + i), or id_map[i] when given,
 // with the op's write flag in bit 31
 __global__ __launch_bounds__(256) void k_probe(IndexView ix, const uint64_t* ops, uint64_t n, uint32_t* rows_out,
                                                uint32_t* rows_copy, uint32_t* ids_out, uint32_t id_base,
                                                const uint32_t* id_map) {
+  if (ix.dense) {  // direct addressing: no bucket to read
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+      const uint64_t op = ops[i];
+      const uint32_t r = idx_dense_find(op_key(op), ix.length);
+      rows_out[i] = r;
+      if (rows_copy) rows_copy[i] = r;
+      if (ids_out) ids_out[i] = (id_map ? id_map[i] : id_base + (uint32_t)i) | (op_is_write(op) ? OPID_WRITE : 0u);
+    }
+    return;
+  }
   const uint32_t q = threadIdx.x & 3u;
   const uint4* bkt = reinterpret_cast<const uint4*>(ix.buckets);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of 4

@@ -230,7 +230,13 @@ struct IndexView {
   const uint32_t* buckets;  // nbuckets * 16 u32: [8 keys][8 values]
   uint32_t nbuckets;
   uint32_t length;          // number of keys; Find's miss value
+  // 1: the keys are 0 .. length-1 and key k is row k (the identity index, a
+  // sorted array of dense keys): a lookup is direct addressing and reads
+  // nothing; buckets is null
+  uint32_t dense;
 };
+// Row of key in a dense index: the key itself, or a miss past the end
+GC_HD uint32_t idx_dense_find(uint32_t key, uint32_t length) { return key < length ? key : IDX_MISS; }
 // Home bucket: Fibonacci hashing (multiply by 2^32 / golden ratio, an odd
 // constant, so distinct keys keep distinct products), mapped onto
 // [0, nbuckets) by the high half of a 32 x 32 product.

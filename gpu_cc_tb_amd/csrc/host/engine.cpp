@@ -121,6 +121,7 @@ struct gcctb_ctx {
   gcctb::DevBuf<uint64_t> cells;
   gcctb::DevBuf<uint32_t> buckets;  // hash index
   gcctb::IndexView ix{};
+  bool index_hash = false;  // GCCTB_INDEX_HASH=1: the identity index as a hash table
   bool table_ready = false;
   // transactions
   uint64_t ntx = 0, nops = 0, nreads = 0;
@@ -617,10 +618,16 @@ int hip_fail(const HipFailure& f) {
   return set_error(GCCTB_EDEVICE, std::string("HIP error: ") + hipGetErrorString(f.err) + " in " + f.where);
 }
 
-// Hash index over n keys (dkeys / dvals device arrays; nullptr: identity).
+bool index_ready(const gcctb_ctx& c) { return c.ix.buckets || c.ix.dense; }
+
+// Index over n keys: the identity (dkeys and dvals null) is a dense index,
+// direct addressing with no table, unless GCCTB_INDEX_HASH=1 asks for the
+// hash table over keys 0 .. n-1; other keys (device arrays) go into the hash
+// table.
 void build_index(gcctb_ctx& c, const uint32_t* dkeys, const uint32_t* dvals, uint64_t n) {
-  const uint64_t nb = idx_buckets_for(n);
-  if (nb > 0xffffffffull) throw HipFailure{hipErrorInvalidValue, "index too large"};
+  const bool dense = !dkeys && !dvals && !c.index_hash;
+  const uint64_t nb = dense ? 0 : idx_buckets_for(n);
+  if (nb > 0xffffffffull || n > 0xffffffffull) throw HipFailure{hipErrorInvalidValue, "index too large"};
   // A captured graph holds the old bucket array in its kernels' arguments:
   // it is rebuilt on its next launch, and dropped now, before the buckets it
   // points to are freed.
@@ -631,6 +638,11 @@ void build_index(gcctb_ctx& c, const uint32_t* dkeys, const uint32_t* dvals, uin
   }
   c.ix = IndexView{};
   c.buckets.release();
+  if (dense) {
+    c.ix.length = (uint32_t)n;
+    c.ix.dense = 1;
+    return;
+  }
   c.buckets.alloc(nb * 2 * IDX_SLOTS);
   HIP_TRY(launch::fill_u32(c.buckets.p, nb * 2 * IDX_SLOTS, IDX_EMPTY, c.stream));
   HIP_TRY(launch::index_insert(dkeys, dvals, n, c.buckets.p, (uint32_t)nb, c.stream));
@@ -724,6 +736,8 @@ int gcctb_create(const gcctb_config* cfg, gcctb_ctx** out) {
   c->G = 1 << cfg->warp_density;
   c->threads = cfg->block_size * 32;
   {
+    const char* ih = std::getenv("GCCTB_INDEX_HASH");
+    c->index_hash = ih && std::atoi(ih) == 1;
     // watchdog: 20 s of the 100 MHz wall clock unless GCCTB_WATCHDOG_S says otherwise
     const char* wd = std::getenv("GCCTB_WATCHDOG_S");
     const double sec = wd ? std::atof(wd) : 20.0;
@@ -834,7 +848,14 @@ int gcctb_build_index(gcctb_ctx* c, const uint32_t* keys, const uint32_t* vals, 
     for (uint64_t i = 1; i < n; ++i)
       if (sv[i] == sv[i - 1]) return set_error(GCCTB_EINVAL, "two index keys map to one row");
   }
+  // keys 0 .. n-1, each on the row of its own number: the dense index
+  bool identity = true;
+  for (uint64_t i = 0; i < n && identity; ++i) identity = k[i] == i && v[i] == i;
   try {
+    if (identity) {
+      build_index(*c, nullptr, nullptr, n);
+      return GCCTB_OK;
+    }
     DevBuf<uint32_t> dk, dv;
     dk.alloc(n);
     dv.alloc(n);
@@ -849,7 +870,7 @@ int gcctb_build_index(gcctb_ctx* c, const uint32_t* keys, const uint32_t* vals, 
 
 int gcctb_probe(gcctb_ctx* c, const uint32_t* keys, uint64_t n, uint32_t* rows_out) {
   if (!c || (n && (!keys || !rows_out))) return set_error(GCCTB_EINVAL, "null argument");
-  if (!c->ix.buckets) return set_error(GCCTB_ESTATE, "no index");
+  if (!index_ready(*c)) return set_error(GCCTB_ESTATE, "no index");
   if (n == 0) return GCCTB_OK;
   try {
     std::vector<uint64_t> ops(n);
@@ -877,7 +898,7 @@ int gcctb_load_table(gcctb_ctx* c, const uint64_t* cells, uint64_t rows, uint32_
   try {
     HIP_TRY(hipMemcpyAsync(c->cells.p, cells, rows * cols * 8, hipMemcpyHostToDevice, c->stream));
     ensure_state(*c);
-    if (!c->ix.buckets) build_index(*c, nullptr, nullptr, rows);
+    if (!index_ready(*c)) build_index(*c, nullptr, nullptr, rows);
     HIP_TRY(hipStreamSynchronize(c->stream));
   } catch (const HipFailure& f) {
     return hip_fail(f);
@@ -891,7 +912,7 @@ int gcctb_seed_table(gcctb_ctx* c) {
   try {
     HIP_TRY(launch::seed_table(c->cells.p, c->cfg.rows, c->cfg.cols, c->cfg.seed, c->stream));
     ensure_state(*c);
-    if (!c->ix.buckets) build_index(*c, nullptr, nullptr, c->cfg.rows);
+    if (!index_ready(*c)) build_index(*c, nullptr, nullptr, c->cfg.rows);
     HIP_TRY(hipStreamSynchronize(c->stream));
   } catch (const HipFailure& f) {
     return hip_fail(f);
