@@ -102,12 +102,16 @@ int gcctb_seed_table(gcctb_ctx* ctx);
 /* Build the index over the id column (key i -> row i): replaces
  * Table_CPU::AddIndex(SORTED_ARRAY, 0) (include/table.cuh:95-130,
  * include/index.cuh:96-111).  The index answers Find's question (value, or
- * the length on a miss) from a hash table of unique keys instead of a sorted
- * array.  Called by load_table/seed_table. */
+ * the length on a miss).  Over these dense keys it is direct addressing (the
+ * row of key k is k, a key >= rows misses) and stores nothing; with
+ * GCCTB_INDEX_HASH=1 it is the hash table that gcctb_build_index builds.
+ * Called by load_table/seed_table. */
 int gcctb_build_index_identity(gcctb_ctx* ctx);
 /* Build the index from host (key, value) pairs (any order; keys distinct,
  * keys < 2^32-1, values < rows and distinct: a primary-key index, so that
  * distinct keys of a transaction name distinct rows; GCCTB_EINVAL otherwise).
+ * A hash table of the pairs; pairs that are exactly keys 0 .. n-1 on rows of
+ * the same number make the dense index of gcctb_build_index_identity.
  * A graph captured by gcctb_run_graph is dropped and captured again on its
  * next call. */
 int gcctb_build_index(gcctb_ctx* ctx, const uint32_t* keys, const uint32_t* vals, uint64_t n);
