@@ -239,9 +239,10 @@ class SchemeHost {
   // rows_out, so the persistent and serving forms need no probe pass of their own.
   virtual bool ProbesRows() const { return false; }
   // Persistent and serving forms: whether a probe pass before the launch
-  // beats looking rows up in the hash index as each transaction is taken
-  // (one bucket read on the refill chain).  Measured on the low-contention
-  // workload: only MVCC gains from the pass.
+  // beats looking rows up in the index as each transaction is taken (one
+  // bucket read on the refill chain over a hash table, none over the dense
+  // index).  Measured on the low-contention workload: only MVCC over a hash
+  // table gains from the pass.
   virtual bool WantsProbePass() const { return false; }
   virtual void Init(int batch_id, uint64_t batch_st) {
     (void)batch_id;
@@ -368,7 +369,7 @@ class MVCCHost : public SchemeHost {
     HIP_TRY(launch::exec_mvcc(ctx.G, a, hi - lo, ctx.threads, persist, ctx.stream));
   }
   uint64_t GetMemSize() const override { return ctx.cc.n * 8 + ctx.vprev.n * 4 + ctx.pool.n * 8; }
-  bool WantsProbePass() const override { return true; }
+  bool WantsProbePass() const override { return !ctx.ix.dense; }
 };
 class SiloHost : public SchemeHost {
  public:
