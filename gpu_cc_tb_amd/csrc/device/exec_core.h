@@ -1529,6 +1529,21 @@ constexpr uint64_t ADMIT_TICKS = 1000;   // a wave whose groups wait for an admi
 constexpr uint64_t STALL_TICKS = 100000; // held-back waves sample the slot counter every 1 ms
 constexpr uint64_t NO_SLOT = ~0ull;
 
+// One round of a scheme, inlined for groups of at least 4 lanes (at most
+// TxCtx<G>::K = 8 op slots per lane).  Called out of line, the group's
+// context would live in scratch memory, and every access to it would go
+// through the stack: the compiler's own choice did that to some
+// instantiations and halved their throughput.  Narrower groups carry up to
+// 32 slots per lane, and there the compiler decides.
+template <class S, int G>
+__device__ __forceinline__ void scheme_round(TxCtx<G>& c, const ExecArgs& a) {
+  if constexpr (G >= 4) {
+    [[clang::always_inline]] S::round(c, a);
+  } else {
+    S::round(c, a);
+  }
+}
+
 template <class S, int G>
 __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a) {
   const bool leader = (lane_id() & (G - 1)) == 0;
@@ -1647,7 +1662,7 @@ __device__ void run_persistent(TxCtx<G>& c, const ExecArgs& a) {
     else if (ballot(!c.done && !((c.ph == PH_START || S::kRestInOps) && c.t_round < c.t_resume)) == 0)
       __builtin_amdgcn_s_sleep(4);  // every live group is resting
     else
-      S::round(c, a);
+      scheme_round<S, G>(c, a);
     c.t_round = now = now_ticks();
     const bool stuck = !c.done && a.watchdog_ticks && now - c.t_start > a.watchdog_ticks;
     if (ballot(stuck) != 0) {
@@ -1678,7 +1693,7 @@ __global__ __launch_bounds__(MAXT) void txn_exec(ExecArgs a) {
       if (ballot(!c.done && !((c.ph == PH_START || S::kRestInOps) && c.t_round < c.t_resume)) == 0)
         __builtin_amdgcn_s_sleep(4);
       else
-        S::round(c, a);
+        scheme_round<S, G>(c, a);
       c.t_round = now_ticks();
       // Watchdog on the oldest live transaction of the wave; the wall clock is
       // a scalar read, so the exit is wave-uniform.
