@@ -12,9 +12,8 @@ generated deterministically.
 A step is one pass of every scheme over all 1,048,576 transactions in the
 batched form: the scheme's preparation (GaccO's and GPUTx's ordering passes;
 the sampled hot-key count of the writer deferral for 2PL, T/O and MVCC), one
-persistent launch over
-the 16 batches whose groups refill as transactions commit, one restore of the
-touched CC words.
+persistent launch over the 16 batches whose groups refill as transactions
+commit, one restore of the touched CC words.
 The timed region holds exactly --steps steps; inputs are already resident in
 HBM.  `value` is whole-job committed Tx/s of the batched form (8 x 1,048,576 x
 steps transactions / time).  The plain form (the reference's per batch
